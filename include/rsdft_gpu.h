@@ -1,0 +1,147 @@
+/*
+ * rsdft_gpu.h -- C-ABI of the MI355X Kohn-Sham inner-loop library.
+ *
+ * Every entry point returns 0 on success and a non-zero status otherwise
+ * (rsdft_gpu_last_error() gives the message).  Arrays are plain host
+ * pointers in the reference's Fortran layout:
+ *   psi(n1:n2, ib1:ib2)   band-major, grid index i1 + N1*(i2 + N2*i3);
+ *   coef_lap(3, Md)       Fortran column-major;
+ *   JJP/uVk(MAXMJJ,nzlma) Fortran column-major, JJP 1-based;
+ *   duVdR(3,MMJJ,nzlma)   Fortran column-major.
+ * One process drives one GPU; the library is not re-entrant.  All work is
+ * ordered on one HIP stream.
+ *
+ * Residency: rsdft_gpu_wf_attach() gives the library a device mirror of a
+ * host wavefunction array.  The operations below that take `unk` operate on
+ * that mirror when `unk` is the attached pointer (no host traffic; the host
+ * copy is refreshed only by rsdft_gpu_wf_sync(0)); any other pointer is
+ * copied in and out around the call.
+ */
+#ifndef RSDFT_GPU_H
+#define RSDFT_GPU_H
+
+#if defined(__GNUC__)
+#define RSDFT_API __attribute__((visibility("default")))
+#else
+#define RSDFT_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision modes for rsdft_gpu_set_precision(). */
+#define RSDFT_PREC_FP64       0  /* FP64 storage, FP64 operators and GEMMs            */
+#define RSDFT_PREC_MIXED_FP32 1  /* FP32 storage/operators, FP32 GEMMs, FP64 small LA */
+#define RSDFT_PREC_MIXED_BF16 2  /* FP32 storage/operators, BF16x3 split MFMA GEMMs   */
+
+/* Grid, FD kinetic coefficients and the cell volume element.
+ * Replaces the set-up done by Init_RgridSol (rgrid_sol_module.f90:63-76),
+ * init_kinetic (kinetic_module.f90:20-153) and init_bcset (bc_module.f90);
+ * the periodic halo becomes in-device modular indexing. */
+RSDFT_API int rsdft_gpu_init(int N1, int N2, int N3, const double *Hgrid, double dV, int Md,
+                   const double *coef_lap, double coef_lap0);
+
+/* Local potential Vloc(ML_0:ML_1, s) (localpot_module.f90:14-21). */
+RSDFT_API int rsdft_gpu_set_vloc(const double *vloc, int s);
+
+/* KB projector tables as prep_ps_nloc2 leaves them
+ * (ps_nloc2_variables.f90:5-27, ps_nloc2_module.f90:912-927). */
+RSDFT_API int rsdft_gpu_set_projectors(int nzlma, int maxmjj, const int *MJJ, const int *JJP,
+                             const double *uVk, const int *iuV);
+
+/* Position-derivative tables for the nonlocal force: amap(nzlma) 1-based atom
+ * of each projector, MJJ_MAP(nzlma), JJ_MAP(MMJJ,nzlma) 1-based grid index of
+ * every sphere point, duVdR(3,MMJJ,nzlma) = d uV / d R_atom
+ * (ps_nloc2_module.f90:1489-1587). */
+RSDFT_API int rsdft_gpu_set_projector_derivatives(int natom, const int *amap, int mmjj,
+                                        const int *MJJ_MAP, const int *JJ_MAP,
+                                        const double *duVdR);
+
+/* RSDFT_PREC_* for every later call. */
+RSDFT_API int rsdft_gpu_set_precision(int mode);
+
+/* Ncg (cg_module.f90:23, keyword NCG) and mloop (cgpc_module.f90:95-96). */
+RSDFT_API int rsdft_gpu_set_solver_params(int ncg, int mloop);
+
+/* hamiltonian(k,s,tpsi,htpsi,n1,n2,ib1,ib2) (hamiltonian_module.f90:20-72):
+ * htpsi = T tpsi + Vloc tpsi + V_NL tpsi (htpsi overwritten).  Gamma only:
+ * k must be 1, s must be 1. */
+RSDFT_API int rsdft_gpu_hamiltonian(int k, int s, const double *tpsi, double *htpsi,
+                          int n1, int n2, int ib1, int ib2);
+
+/* op_kinetic (kinetic_module.f90:186-202 -> kinetic_sol_module.f90:20-149):
+ * OVERWRITES htpsi. */
+RSDFT_API int rsdft_gpu_op_kinetic(int k, const double *tpsi, double *htpsi,
+                         int n1, int n2, int ib1, int ib2);
+
+/* op_localpot(s,mm,nn,f,vf) (localpot_module.f90:27-81): vf += Vloc*f. */
+RSDFT_API int rsdft_gpu_op_localpot(int s, int mm, int nn, const double *f, double *vf);
+
+/* op_nonlocal (nonlocal_module.f90:32-48 -> ps_nloc2_op_module.f90:136-270):
+ * htpsi += V_NL tpsi. */
+RSDFT_API int rsdft_gpu_op_nonlocal(int k, int s, const double *tpsi, double *htpsi,
+                          int n1, int n2, int ib1, int ib2);
+
+/* Device mirror of unk(n1:n2, 1:MB) (wf_module.f90:118).  dir 0: device to
+ * host, dir 1: host to device. */
+RSDFT_API int rsdft_gpu_wf_attach(double *unk, int ML0, int MB);
+RSDFT_API int rsdft_gpu_wf_sync(int dir);
+RSDFT_API int rsdft_gpu_wf_detach(void);
+
+/* conjugate_gradient(n1,n2,MB,k,s,unk,esp,res) (cg_module.f90:54-94,
+ * conjugate_gradient_1 :97-430 with precond_hprsdft
+ * cgpc_hprsdft_module.f90:126-348), all bands batched on the device. */
+RSDFT_API int rsdft_gpu_conjugate_gradient(int n1, int n2, int MB, int k, int s,
+                                 double *unk, double *esp, double *res);
+
+/* gram_schmidt(n0,n1,k,s) (gram_schmidt_module.f90:28-55 ->
+ * gram_schmidt_t_module.f90:84-157): orthonormalises all MB bands of unk
+ * with respect to dV (the reference ignores n0, n1 likewise). */
+RSDFT_API int rsdft_gpu_gram_schmidt(double *unk, int ML0, int MB, int n0, int n1);
+
+/* subspace_diag(k,s) (subspace_diag_module.f90:17-25 ->
+ * subspace_diag_la_module.f90:18-232): Rayleigh-Ritz in place, esp(1:MB)
+ * ascending. */
+RSDFT_API int rsdft_gpu_subspace_diag(double *unk, double *esp, int ML0, int MB);
+
+/* nsweep iterations of the calc_sweep body (sweep_module.f90:147-158):
+ * CG -> GS -> subspace_diag, with esp/res of the last sweep returned. */
+RSDFT_API int rsdft_gpu_sweep(double *unk, double *esp, double *res, int ML0, int MB, int nsweep);
+
+/* calc_density, NCPP (density_module.f90:129-136): rho = sum occ psi^2. */
+RSDFT_API int rsdft_gpu_calc_density(const double *unk, int ML0, int MB, const double *occ, double *rho);
+
+/* calc_force_ps_nloc2 for real psi on one rank (ps_nloc2_module.f90:1281-1811):
+ * force2(3, natom) = -dE/dR of the nonlocal band energy at fixed psi. */
+RSDFT_API int rsdft_gpu_force_ps_nloc2(const double *unk, int ML0, int MB, const double *occ,
+                             double *force2);
+
+/* Backward entry of E = sum_n occ_n <psi_n|H|psi_n> dV at fixed psi:
+ * *E, dEdV(ML) = dV sum_n occ_n psi_n^2 (the density_module.f90:129-136
+ * density times dV) and dEdR(3, natom) = -force2.  Any output may be NULL. */
+RSDFT_API int rsdft_gpu_band_energy_grad(const double *unk, int ML0, int MB, const double *occ,
+                               double *E, double *dEdV, double *dEdR);
+
+/* test_hpsi2 (test_hpsi2_module.f90:16-143): times nrep applications of
+ * hamiltonian, op_kinetic, op_localpot, op_nonlocal to the attached
+ * wavefunctions; times_ms[4] = mean milliseconds per full-band application. */
+RSDFT_API int rsdft_gpu_test_hpsi(int nrep, double *times_ms);
+
+/* Per-phase timers in milliseconds since the last reset (watch_module.f90
+ * time_hmlt/time_cgpc analogue): [0] hamiltonian, [1] preconditioner,
+ * [2] CG total, [3] Gram-Schmidt, [4] subspace_diag, [5] sweep total,
+ * [6] number of band-hamiltonian applications.  n <= 8. */
+RSDFT_API int rsdft_gpu_get_timers(double *t, int n);
+RSDFT_API int rsdft_gpu_reset_timers(void);
+
+/* Release every device resource. */
+RSDFT_API int rsdft_gpu_finalize(void);
+
+/* Message of the last failing call (static storage). */
+RSDFT_API const char *rsdft_gpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RSDFT_GPU_H */

@@ -1,0 +1,694 @@
+// C-ABI entry points (include/rsdft_gpu.h): context, tables, residency and
+// the drivers that replace hamiltonian / op_* / conjugate_gradient /
+// gram_schmidt / subspace_diag / calc_sweep / calc_density /
+// calc_force_ps_nloc2 of the reference.
+#include "common.h"
+#include "rsdft_gpu.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+namespace rsdft {
+
+static char g_err[1024] = "";
+
+int set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+const char *last_error() { return g_err; }
+
+int DevMem::ensure(size_t nbytes)
+{
+    if (nbytes <= bytes && p) return kOK;
+    release();
+    if (nbytes == 0) return kOK;
+    hipError_t e = hipMalloc(&p, nbytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return set_error(kErrHip, "hipMalloc(%zu) -> %s", nbytes, hipGetErrorString(e));
+    }
+    bytes = nbytes;
+    return kOK;
+}
+
+static Context g_ctx;
+Context &ctx() { return g_ctx; }
+
+// ---- timers ---------------------------------------------------------------
+struct PhaseEvents {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<int> slot;
+    int begin(int s) {
+        hipEvent_t a, b;
+        RS_HIP(hipEventCreate(&a));
+        RS_HIP(hipEventCreate(&b));
+        RS_HIP(hipEventRecord(a, ctx().stream));
+        ev.push_back({a, b});
+        slot.push_back(s);
+        return kOK;
+    }
+    int end() {
+        RS_HIP(hipEventRecord(ev.back().second, ctx().stream));
+        return kOK;
+    }
+    int flush() {   // stream must be synchronised
+        for (size_t k = 0; k < ev.size(); ++k) {
+            float ms = 0.0f;
+            RS_HIP(hipEventElapsedTime(&ms, ev[k].first, ev[k].second));
+            ctx().timers.ms[slot[k]] += ms;
+            (void)hipEventDestroy(ev[k].first);
+            (void)hipEventDestroy(ev[k].second);
+        }
+        ev.clear();
+        slot.clear();
+        return kOK;
+    }
+};
+
+enum { kTHam = 0, kTPc = 1, kTCg = 2, kTGs = 3, kTSd = 4, kTSweep = 5 };
+
+// ---- helpers --------------------------------------------------------------
+static int need_init()
+{
+    if (!ctx().init) return set_error(kErrNotInit, "rsdft_gpu_init has not been called");
+    return kOK;
+}
+
+static size_t elem_bytes(int prec) { return prec == 0 ? 8 : 4; }
+
+static int ensure_wf_buffers(int MB)
+{
+    Context &c = ctx();
+    const size_t n = (size_t)c.g.ML * MB * elem_bytes(c.prec);
+    DevMem *bufs[] = {&c.psi, &c.hx, &c.hp, &c.gk, &c.pgk, &c.pk, &c.rk, &c.ppc, &c.qpc, &c.tmp};
+    if (c.wf_prec != c.prec) {
+        for (DevMem *b : bufs) b->release();
+    }
+    for (DevMem *b : bufs) RS_TRY(b->ensure(n));
+    RS_TRY(c.dots.ensure(sizeof(double) * 4 * (size_t)MB));
+    RS_TRY(c.cg.ensure(MB));
+    c.wf_prec = c.prec;
+    c.MB = MB;
+    return kOK;
+}
+
+// host FP64 -> device mirror c.psi (working precision)
+static int upload_psi(const double *unk, int MB)
+{
+    Context &c = ctx();
+    const size_t n = (size_t)c.g.ML * MB;
+    if (c.prec == 0) {
+        RS_HIP(hipMemcpyAsync(c.psi.p, unk, n * 8, hipMemcpyHostToDevice, c.stream));
+    } else {
+        RS_TRY(c.stage.ensure(n * 8));
+        RS_HIP(hipMemcpyAsync(c.stage.p, unk, n * 8, hipMemcpyHostToDevice, c.stream));
+        RS_TRY(convert_from_f64<float>(c.stage.as<double>(), c.psi.as<float>(), (long)n));
+    }
+    return kOK;
+}
+
+static int download_psi(double *unk, int MB)
+{
+    Context &c = ctx();
+    const size_t n = (size_t)c.g.ML * MB;
+    if (c.prec == 0) {
+        RS_HIP(hipMemcpyAsync(unk, c.psi.p, n * 8, hipMemcpyDeviceToHost, c.stream));
+    } else {
+        RS_TRY(c.stage.ensure(n * 8));
+        RS_TRY(convert_to_f64<float>(c.psi.as<float>(), c.stage.as<double>(), (long)n));
+        RS_HIP(hipMemcpyAsync(unk, c.stage.p, n * 8, hipMemcpyDeviceToHost, c.stream));
+    }
+    RS_HIP(hipStreamSynchronize(c.stream));
+    return kOK;
+}
+
+// Make c.psi hold unk(ML0, MB): either it is the attached array (nothing to
+// do) or it is copied in.  *staged tells the caller to copy back.
+static int bind_wf(const double *unk, int ML0, int MB, bool *staged)
+{
+    Context &c = ctx();
+    if (ML0 != c.g.ML) return set_error(kErrArg, "ML0=%d but the grid has %ld points", ML0, c.g.ML);
+    if (MB <= 0) return set_error(kErrArg, "MB=%d", MB);
+    if (unk == c.host_unk && MB == c.MB && c.wf_prec == c.prec) {
+        *staged = false;
+        return kOK;
+    }
+    c.host_unk = nullptr;
+    RS_TRY(ensure_wf_buffers(MB));
+    RS_TRY(upload_psi(unk, MB));
+    *staged = true;
+    return kOK;
+}
+
+static int check_solver_flags()
+{
+    Context &c = ctx();
+    int info[4] = {0, 0, 0, 0};
+    RS_HIP(hipMemcpy(info, c.einfo.p, sizeof(info), hipMemcpyDeviceToHost));
+    if (info[2] != 0) return set_error(kErrSolver, "Cholesky of the overlap failed (info=%d)", info[2]);
+    if (info[3] != 0) return set_error(kErrSolver, "subspace eigensolve failed (info=%d)", info[3]);
+    return kOK;
+}
+
+static int clear_solver_flags()
+{
+    Context &c = ctx();
+    RS_TRY(c.einfo.ensure(sizeof(int) * 4));
+    RS_HIP(hipMemsetAsync(c.einfo.p, 0, sizeof(int) * 4, c.stream));
+    return kOK;
+}
+
+template <class T> static T *psiT() { return ctx().psi.as<T>(); }
+
+template <class T>
+static int run_sweeps(int MB, int nsweep, double *esp_dev, double *res_dev, PhaseEvents &pe)
+{
+    for (int it = 0; it < nsweep; ++it) {
+        RS_TRY(pe.begin(kTSweep));
+        RS_TRY(pe.begin(kTCg));
+        RS_TRY(cg_batched<T>(psiT<T>(), MB, esp_dev, res_dev));
+        RS_TRY(pe.end());
+        RS_TRY(pe.begin(kTGs));
+        RS_TRY(gram_schmidt_dev<T>(psiT<T>(), MB));
+        RS_TRY(pe.end());
+        RS_TRY(pe.begin(kTSd));
+        RS_TRY(subspace_diag_dev<T>(psiT<T>(), MB, esp_dev));
+        RS_TRY(pe.end());
+        // close the sweep pair (it was opened first)
+        RS_HIP(hipEventRecord(pe.ev[pe.ev.size() - 4].second, ctx().stream));
+    }
+    return kOK;
+}
+
+}  // namespace rsdft
+
+using namespace rsdft;
+
+extern "C" {
+
+const char *rsdft_gpu_last_error(void) { return last_error(); }
+
+int rsdft_gpu_init(int N1, int N2, int N3, const double *Hgrid, double dV, int Md,
+                   const double *coef_lap, double coef_lap0)
+{
+    Context &c = ctx();
+    if (N1 <= 0 || N2 <= 0 || N3 <= 0 || !Hgrid || !coef_lap)
+        return set_error(kErrArg, "bad grid arguments");
+    if (Md < 1 || Md > kMaxMd) return set_error(kErrArg, "Md=%d out of range", Md);
+    if (N1 % 4) return set_error(kErrArg, "N1=%d must be a multiple of 4", N1);
+    if (N1 < 2 * Md + 1 || N2 < 2 * Md + 1 || N3 < 2 * Md + 1)
+        return set_error(kErrArg, "grid smaller than the stencil");
+    if (!c.init) {
+        RS_HIP(hipGetDevice(&c.device));
+        RS_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        RS_BLAS(rocblas_create_handle(&c.blas));
+        RS_BLAS(rocblas_set_stream(c.blas, c.stream));
+        RS_BLAS(rocblas_set_atomics_mode(c.blas, rocblas_atomics_not_allowed));
+    }
+    c.g = GridDesc();
+    c.g.N1 = N1;
+    c.g.N2 = N2;
+    c.g.N3 = N3;
+    c.g.Md = Md;
+    c.g.ML = (long)N1 * N2 * N3;
+    for (int d = 0; d < 3; ++d) c.g.h[d] = Hgrid[d];
+    c.g.dV = dV;
+    c.g.coef_lap0 = coef_lap0;
+    for (int m = 0; m < Md; ++m)
+        for (int d = 0; d < 3; ++d) c.g.coef_lap[d][m] = coef_lap[d + 3 * m];
+    c.geom_h = choose_geom(N1, N2, N3, Md, 4, 8);
+    c.geom_pc = choose_geom(N1, N2, N3, 1, 4, 8);
+    c.host_unk = nullptr;
+    c.MB = 0;
+    c.init = true;
+    return kOK;
+}
+
+int rsdft_gpu_set_vloc(const double *vloc, int s)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    if (s != 1) return set_error(kErrArg, "spin index %d: one spin channel only", s);
+    const size_t n = (size_t)c.g.ML;
+    RS_TRY(c.vloc64.ensure(n * 8));
+    RS_TRY(c.vloc32.ensure(n * 4));
+    RS_HIP(hipMemcpyAsync(c.vloc64.p, vloc, n * 8, hipMemcpyHostToDevice, c.stream));
+    RS_TRY(convert_from_f64<float>(c.vloc64.as<double>(), c.vloc32.as<float>(), (long)n));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    return kOK;
+}
+
+int rsdft_gpu_set_projectors(int nzlma, int maxmjj, const int *MJJ, const int *JJP,
+                             const double *uVk, const int *iuV)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    NonlocalTables &nl = c.nl;
+    const long ML = c.g.ML;
+    nl.nz = nzlma;
+    if (nzlma <= 0) { nl.nnz = 0; return kOK; }
+    std::vector<int> rptr(nzlma + 1, 0);
+    for (int l = 0; l < nzlma; ++l) {
+        if (MJJ[l] < 0 || MJJ[l] > maxmjj) return set_error(kErrArg, "MJJ(%d)=%d", l + 1, MJJ[l]);
+        rptr[l + 1] = rptr[l] + MJJ[l];
+    }
+    const long nnz = rptr[nzlma];
+    std::vector<int> ridx(nnz);
+    std::vector<double> rval(nnz);
+    std::vector<int> cptr(ML + 1, 0);
+    for (int l = 0; l < nzlma; ++l)
+        for (int j = 0; j < MJJ[l]; ++j) {
+            const int gi = JJP[(long)l * maxmjj + j] - 1;
+            if (gi < 0 || gi >= ML) return set_error(kErrArg, "JJP(%d,%d)=%d out of grid", j + 1, l + 1, gi + 1);
+            ridx[rptr[l] + j] = gi;
+            rval[rptr[l] + j] = uVk[(long)l * maxmjj + j];
+            cptr[gi + 1]++;
+        }
+    for (long i = 0; i < ML; ++i) cptr[i + 1] += cptr[i];
+    std::vector<int> fill(cptr.begin(), cptr.end() - 1), clma(nnz);
+    std::vector<double> cval(nnz);
+    // column entries in lma order per point: the reference adds projector
+    // contributions in lma order (ps_nloc2_op_module.f90:255-263)
+    for (int l = 0; l < nzlma; ++l)
+        for (int j = 0; j < MJJ[l]; ++j) {
+            const int gi = ridx[rptr[l] + j];
+            const int e = fill[gi]++;
+            clma[e] = l;
+            cval[e] = rval[rptr[l] + j];
+        }
+    std::vector<float> rval32(rval.begin(), rval.end()), cval32(cval.begin(), cval.end());
+    std::vector<double> scale(nzlma);
+    for (int l = 0; l < nzlma; ++l) scale[l] = (double)iuV[l] * c.g.dV;
+    nl.nnz = nnz;
+    RS_TRY(nl.row_ptr.ensure(sizeof(int) * (nzlma + 1)));
+    RS_TRY(nl.row_idx.ensure(sizeof(int) * nnz));
+    RS_TRY(nl.row_val64.ensure(sizeof(double) * nnz));
+    RS_TRY(nl.row_val32.ensure(sizeof(float) * nnz));
+    RS_TRY(nl.col_ptr.ensure(sizeof(int) * (ML + 1)));
+    RS_TRY(nl.col_lma.ensure(sizeof(int) * nnz));
+    RS_TRY(nl.col_val64.ensure(sizeof(double) * nnz));
+    RS_TRY(nl.col_val32.ensure(sizeof(float) * nnz));
+    RS_TRY(nl.scale64.ensure(sizeof(double) * nzlma));
+    RS_HIP(hipMemcpy(nl.row_ptr.p, rptr.data(), sizeof(int) * (nzlma + 1), hipMemcpyHostToDevice));
+    RS_HIP(hipMemcpy(nl.row_idx.p, ridx.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+    RS_HIP(hipMemcpy(nl.row_val64.p, rval.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
+    RS_HIP(hipMemcpy(nl.row_val32.p, rval32.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
+    RS_HIP(hipMemcpy(nl.col_ptr.p, cptr.data(), sizeof(int) * (ML + 1), hipMemcpyHostToDevice));
+    RS_HIP(hipMemcpy(nl.col_lma.p, clma.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+    RS_HIP(hipMemcpy(nl.col_val64.p, cval.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
+    RS_HIP(hipMemcpy(nl.col_val32.p, cval32.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
+    RS_HIP(hipMemcpy(nl.scale64.p, scale.data(), sizeof(double) * nzlma, hipMemcpyHostToDevice));
+    return kOK;
+}
+
+int rsdft_gpu_set_projector_derivatives(int natom, const int *amap, int mmjj, const int *MJJ_MAP,
+                                        const int *JJ_MAP, const double *duVdR)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    NonlocalTables &nl = c.nl;
+    const int nz = nl.nz;
+    if (nz <= 0) return set_error(kErrState, "set projectors first");
+    if (natom <= 0) return set_error(kErrArg, "natom=%d", natom);
+    std::vector<int> dptr(nz + 1, 0);
+    for (int l = 0; l < nz; ++l) {
+        if (MJJ_MAP[l] < 0 || MJJ_MAP[l] > mmjj) return set_error(kErrArg, "MJJ_MAP(%d)", l + 1);
+        dptr[l + 1] = dptr[l] + MJJ_MAP[l];
+    }
+    const long nnz = dptr[nz];
+    std::vector<int> didx(nnz);
+    std::vector<double> dval(3 * nnz);
+    for (int l = 0; l < nz; ++l)
+        for (int j = 0; j < MJJ_MAP[l]; ++j) {
+            const long e = dptr[l] + j;
+            const int gi = JJ_MAP[(long)l * mmjj + j] - 1;
+            if (gi < 0 || gi >= c.g.ML) return set_error(kErrArg, "JJ_MAP out of grid");
+            didx[e] = gi;
+            for (int d = 0; d < 3; ++d) dval[3 * e + d] = duVdR[3 * ((long)l * mmjj + j) + d];
+        }
+    // atom -> projector CSR: [aptr(natom+1) | alma(nz)]
+    std::vector<int> cnt(natom + 1, 0), tab(natom + 1 + nz);
+    for (int l = 0; l < nz; ++l) {
+        const int a = amap[l] - 1;
+        if (a < 0 || a >= natom) return set_error(kErrArg, "amap(%d)=%d", l + 1, amap[l]);
+        cnt[a + 1]++;
+    }
+    for (int a = 0; a < natom; ++a) cnt[a + 1] += cnt[a];
+    for (int a = 0; a <= natom; ++a) tab[a] = cnt[a];
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int l = 0; l < nz; ++l) tab[natom + 1 + fill[amap[l] - 1]++] = l;
+    nl.natom = natom;
+    nl.nnzd = nnz;
+    RS_TRY(nl.d_ptr.ensure(sizeof(int) * (nz + 1)));
+    RS_TRY(nl.d_idx.ensure(sizeof(int) * std::max<long>(nnz, 1)));
+    RS_TRY(nl.d_val.ensure(sizeof(double) * 3 * std::max<long>(nnz, 1)));
+    RS_TRY(nl.amap_dev.ensure(sizeof(int) * tab.size()));
+    RS_HIP(hipMemcpy(nl.d_ptr.p, dptr.data(), sizeof(int) * (nz + 1), hipMemcpyHostToDevice));
+    if (nnz) {
+        RS_HIP(hipMemcpy(nl.d_idx.p, didx.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+        RS_HIP(hipMemcpy(nl.d_val.p, dval.data(), sizeof(double) * 3 * nnz, hipMemcpyHostToDevice));
+    }
+    RS_HIP(hipMemcpy(nl.amap_dev.p, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+    return kOK;
+}
+
+int rsdft_gpu_set_precision(int mode)
+{
+    RS_TRY(need_init());
+    if (mode < 0 || mode > 2) return set_error(kErrArg, "precision mode %d", mode);
+    Context &c = ctx();
+    if (c.host_unk && c.wf_prec >= 0 && elem_bytes(mode) != elem_bytes(c.wf_prec))
+        return set_error(kErrState, "detach (after wf_sync) before changing the storage precision");
+    c.prec = mode;
+    if (c.wf_prec >= 0 && elem_bytes(mode) == elem_bytes(c.wf_prec)) c.wf_prec = mode;
+    return kOK;
+}
+
+int rsdft_gpu_set_solver_params(int ncg, int mloop)
+{
+    if (ncg < 0 || mloop < 0) return set_error(kErrArg, "ncg=%d mloop=%d", ncg, mloop);
+    ctx().ncg = ncg;
+    ctx().mloop = mloop;
+    return kOK;
+}
+
+static int op_entry(int k, int s, const double *tpsi, double *htpsi, int n1, int n2, int ib1,
+                    int ib2, int flags)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    if (k != 1) return set_error(kErrArg, "k=%d: Gamma point only", k);
+    if (s != 1) return set_error(kErrArg, "s=%d: one spin channel only", s);
+    const long ML0 = (long)n2 - n1 + 1;
+    const int nb = ib2 - ib1 + 1;
+    if (ML0 != c.g.ML) return set_error(kErrArg, "n2-n1+1=%ld but the grid has %ld points", ML0, c.g.ML);
+    if (nb <= 0) return kOK;
+    const size_t n = (size_t)ML0 * nb;
+    RS_TRY(c.stage.ensure(2 * n * 8));
+    double *din = c.stage.as<double>(), *dout = din + n;
+    RS_HIP(hipMemcpyAsync(din, tpsi, n * 8, hipMemcpyHostToDevice, c.stream));
+    if (flags & kAcc) RS_HIP(hipMemcpyAsync(dout, htpsi, n * 8, hipMemcpyHostToDevice, c.stream));
+    if (c.prec == 0) {
+        RS_TRY(hpsi_apply<double>(din, dout, nb, flags));
+    } else {
+        RS_TRY(c.opbuf.ensure(2 * n * 4));
+        float *fin = c.opbuf.as<float>(), *fout = fin + n;
+        RS_TRY(convert_from_f64<float>(din, fin, (long)n));
+        if (flags & kAcc) RS_TRY(convert_from_f64<float>(dout, fout, (long)n));
+        RS_TRY(hpsi_apply<float>(fin, fout, nb, flags));
+        RS_TRY(convert_to_f64<float>(fout, dout, (long)n));
+    }
+    RS_HIP(hipMemcpyAsync(htpsi, dout, n * 8, hipMemcpyDeviceToHost, c.stream));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    return kOK;
+}
+
+int rsdft_gpu_hamiltonian(int k, int s, const double *tpsi, double *htpsi, int n1, int n2,
+                          int ib1, int ib2)
+{
+    return op_entry(k, s, tpsi, htpsi, n1, n2, ib1, ib2, kKin | kLoc | kNl);
+}
+
+int rsdft_gpu_op_kinetic(int k, const double *tpsi, double *htpsi, int n1, int n2, int ib1, int ib2)
+{
+    return op_entry(k, 1, tpsi, htpsi, n1, n2, ib1, ib2, kKin);
+}
+
+int rsdft_gpu_op_localpot(int s, int mm, int nn, const double *f, double *vf)
+{
+    return op_entry(1, s, f, vf, 1, mm, 1, nn, kLoc | kAcc);
+}
+
+int rsdft_gpu_op_nonlocal(int k, int s, const double *tpsi, double *htpsi, int n1, int n2,
+                          int ib1, int ib2)
+{
+    return op_entry(k, s, tpsi, htpsi, n1, n2, ib1, ib2, kNl | kAcc);
+}
+
+int rsdft_gpu_wf_attach(double *unk, int ML0, int MB)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    if (!unk) return set_error(kErrArg, "null unk");
+    if (ML0 != c.g.ML) return set_error(kErrArg, "ML0=%d but the grid has %ld points", ML0, c.g.ML);
+    RS_TRY(ensure_wf_buffers(MB));
+    RS_TRY(upload_psi(unk, MB));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    c.host_unk = unk;
+    return kOK;
+}
+
+int rsdft_gpu_wf_sync(int dir)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    if (!c.host_unk) return set_error(kErrState, "no wavefunction attached");
+    if (dir == 0) return download_psi(c.host_unk, c.MB);
+    RS_TRY(upload_psi(c.host_unk, c.MB));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    return kOK;
+}
+
+int rsdft_gpu_wf_detach(void)
+{
+    ctx().host_unk = nullptr;
+    return kOK;
+}
+
+int rsdft_gpu_conjugate_gradient(int n1, int n2, int MB, int k, int s, double *unk, double *esp,
+                                 double *res)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    if (k != 1 || s != 1) return set_error(kErrArg, "Gamma point, one spin only");
+    bool staged = false;
+    RS_TRY(bind_wf(unk, n2 - n1 + 1, MB, &staged));
+    PhaseEvents pe;
+    double *ed = c.dots.as<double>();
+    RS_TRY(pe.begin(kTCg));
+    if (c.prec == 0) RS_TRY(cg_batched<double>(psiT<double>(), MB, ed, ed + MB));
+    else RS_TRY(cg_batched<float>(psiT<float>(), MB, ed, ed + MB));
+    RS_TRY(pe.end());
+    RS_HIP(hipMemcpyAsync(esp, ed, sizeof(double) * MB, hipMemcpyDeviceToHost, c.stream));
+    RS_HIP(hipMemcpyAsync(res, ed + MB, sizeof(double) * MB, hipMemcpyDeviceToHost, c.stream));
+    if (staged) RS_TRY(download_psi(unk, MB));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    return pe.flush();
+}
+
+int rsdft_gpu_gram_schmidt(double *unk, int ML0, int MB, int n0, int n1)
+{
+    (void)n0;
+    (void)n1;
+    RS_TRY(need_init());
+    Context &c = ctx();
+    bool staged = false;
+    RS_TRY(bind_wf(unk, ML0, MB, &staged));
+    RS_TRY(clear_solver_flags());
+    PhaseEvents pe;
+    RS_TRY(pe.begin(kTGs));
+    if (c.prec == 0) RS_TRY(gram_schmidt_dev<double>(psiT<double>(), MB));
+    else RS_TRY(gram_schmidt_dev<float>(psiT<float>(), MB));
+    RS_TRY(pe.end());
+    if (staged) RS_TRY(download_psi(unk, MB));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    RS_TRY(pe.flush());
+    return check_solver_flags();
+}
+
+int rsdft_gpu_subspace_diag(double *unk, double *esp, int ML0, int MB)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    bool staged = false;
+    RS_TRY(bind_wf(unk, ML0, MB, &staged));
+    RS_TRY(clear_solver_flags());
+    PhaseEvents pe;
+    double *ed = c.dots.as<double>();
+    RS_TRY(pe.begin(kTSd));
+    if (c.prec == 0) RS_TRY(subspace_diag_dev<double>(psiT<double>(), MB, ed));
+    else RS_TRY(subspace_diag_dev<float>(psiT<float>(), MB, ed));
+    RS_TRY(pe.end());
+    RS_HIP(hipMemcpyAsync(esp, ed, sizeof(double) * MB, hipMemcpyDeviceToHost, c.stream));
+    if (staged) RS_TRY(download_psi(unk, MB));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    RS_TRY(pe.flush());
+    return check_solver_flags();
+}
+
+int rsdft_gpu_sweep(double *unk, double *esp, double *res, int ML0, int MB, int nsweep)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    bool staged = false;
+    RS_TRY(bind_wf(unk, ML0, MB, &staged));
+    RS_TRY(clear_solver_flags());
+    PhaseEvents pe;
+    double *ed = c.dots.as<double>();
+    if (c.prec == 0) RS_TRY(run_sweeps<double>(MB, nsweep, ed, ed + MB, pe));
+    else RS_TRY(run_sweeps<float>(MB, nsweep, ed, ed + MB, pe));
+    if (esp) RS_HIP(hipMemcpyAsync(esp, ed, sizeof(double) * MB, hipMemcpyDeviceToHost, c.stream));
+    if (res) RS_HIP(hipMemcpyAsync(res, ed + MB, sizeof(double) * MB, hipMemcpyDeviceToHost, c.stream));
+    if (staged) RS_TRY(download_psi(unk, MB));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    RS_TRY(pe.flush());
+    return check_solver_flags();
+}
+
+int rsdft_gpu_calc_density(const double *unk, int ML0, int MB, const double *occ, double *rho)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    bool staged = false;
+    RS_TRY(bind_wf(unk, ML0, MB, &staged));
+    RS_TRY(c.stage.ensure(sizeof(double) * (c.g.ML + MB)));
+    double *rd = c.stage.as<double>(), *od = rd + c.g.ML;
+    RS_HIP(hipMemcpyAsync(od, occ, sizeof(double) * MB, hipMemcpyHostToDevice, c.stream));
+    if (c.prec == 0) RS_TRY(density_dev<double>(psiT<double>(), MB, od, rd));
+    else RS_TRY(density_dev<float>(psiT<float>(), MB, od, rd));
+    RS_HIP(hipMemcpyAsync(rho, rd, sizeof(double) * c.g.ML, hipMemcpyDeviceToHost, c.stream));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    if (staged) c.host_unk = nullptr;
+    return kOK;
+}
+
+int rsdft_gpu_force_ps_nloc2(const double *unk, int ML0, int MB, const double *occ, double *force2)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    bool staged = false;
+    RS_TRY(bind_wf(unk, ML0, MB, &staged));
+    const int na = c.nl.natom;
+    if (na <= 0) return set_error(kErrState, "projector derivative tables not set");
+    RS_TRY(c.dots.ensure(sizeof(double) * ((size_t)4 * MB + 3 * na + MB)));
+    double *fd = c.dots.as<double>() + 4 * (size_t)MB;
+    double *od = fd + 3 * na;
+    RS_HIP(hipMemcpyAsync(od, occ, sizeof(double) * MB, hipMemcpyHostToDevice, c.stream));
+    if (c.prec == 0) RS_TRY(force_dev<double>(psiT<double>(), MB, od, fd));
+    else RS_TRY(force_dev<float>(psiT<float>(), MB, od, fd));
+    RS_HIP(hipMemcpyAsync(force2, fd, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, c.stream));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    return kOK;
+}
+
+int rsdft_gpu_band_energy_grad(const double *unk, int ML0, int MB, const double *occ, double *E,
+                               double *dEdV, double *dEdR)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    bool staged = false;
+    RS_TRY(bind_wf(unk, ML0, MB, &staged));
+    const long ML = c.g.ML;
+    if (E) {
+        std::vector<double> be(MB);
+        double *bd = c.dots.as<double>();
+        if (c.prec == 0) {
+            RS_TRY(hpsi_apply<double>(psiT<double>(), c.hp.as<double>(), MB, kKin | kLoc | kNl));
+            const double *a[1] = {psiT<double>()}, *b[1] = {c.hp.as<double>()};
+            RS_TRY(band_dots<double>(1, a, b, MB, c.g.dV, bd, nullptr));
+        } else {
+            RS_TRY(hpsi_apply<float>(psiT<float>(), c.hp.as<float>(), MB, kKin | kLoc | kNl));
+            const float *a[1] = {psiT<float>()}, *b[1] = {c.hp.as<float>()};
+            RS_TRY(band_dots<float>(1, a, b, MB, c.g.dV, bd, nullptr));
+        }
+        RS_HIP(hipMemcpyAsync(be.data(), bd, sizeof(double) * MB, hipMemcpyDeviceToHost, c.stream));
+        RS_HIP(hipStreamSynchronize(c.stream));
+        double e = 0.0;
+        for (int n = 0; n < MB; ++n) e += occ[n] * be[n];
+        *E = e;
+    }
+    if (dEdV) {
+        RS_TRY(rsdft_gpu_calc_density(c.host_unk ? c.host_unk : unk, ML0, MB, occ, dEdV));
+        for (long i = 0; i < ML; ++i) dEdV[i] *= c.g.dV;
+    }
+    if (dEdR) {
+        RS_TRY(rsdft_gpu_force_ps_nloc2(c.host_unk ? c.host_unk : unk, ML0, MB, occ, dEdR));
+        for (int k = 0; k < 3 * c.nl.natom; ++k) dEdR[k] = -dEdR[k];
+    }
+    if (staged) c.host_unk = nullptr;
+    return kOK;
+}
+
+int rsdft_gpu_test_hpsi(int nrep, double *times_ms)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    if (!c.host_unk) return set_error(kErrState, "attach wavefunctions first");
+    const int flags[4] = {kKin | kLoc | kNl, kKin, kLoc | kAcc, kNl | kAcc};
+    hipEvent_t a, b;
+    RS_HIP(hipEventCreate(&a));
+    RS_HIP(hipEventCreate(&b));
+    for (int t = 0; t < 4; ++t) {
+        // one untimed application, then nrep timed
+        for (int rep = -1; rep < nrep; ++rep) {
+            if (rep == 0) RS_HIP(hipEventRecord(a, c.stream));
+            if (c.prec == 0) RS_TRY(hpsi_apply<double>(psiT<double>(), c.hp.as<double>(), c.MB, flags[t]));
+            else RS_TRY(hpsi_apply<float>(psiT<float>(), c.hp.as<float>(), c.MB, flags[t]));
+        }
+        RS_HIP(hipEventRecord(b, c.stream));
+        RS_HIP(hipEventSynchronize(b));
+        float ms = 0.0f;
+        RS_HIP(hipEventElapsedTime(&ms, a, b));
+        if (times_ms) times_ms[t] = nrep > 0 ? ms / nrep : 0.0;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return kOK;
+}
+
+int rsdft_gpu_get_timers(double *t, int n)
+{
+    Context &c = ctx();
+    if (n > 8) n = 8;
+    for (int k = 0; k < n && k < 6; ++k) t[k] = c.timers.ms[k];
+    if (n > 6) t[6] = (double)c.hpsi_bands;
+    if (n > 7) t[7] = 0.0;
+    return kOK;
+}
+
+int rsdft_gpu_reset_timers(void)
+{
+    Context &c = ctx();
+    for (double &x : c.timers.ms) x = 0.0;
+    c.hpsi_bands = 0;
+    return kOK;
+}
+
+int rsdft_gpu_finalize(void)
+{
+    Context &c = ctx();
+    if (!c.init) return kOK;
+    (void)hipStreamSynchronize(c.stream);
+    DevMem *all[] = {&c.vloc64, &c.vloc32, &c.psi, &c.hx, &c.hp, &c.gk, &c.pgk, &c.pk, &c.rk,
+                     &c.ppc, &c.qpc, &c.tmp, &c.coefC, &c.part, &c.stage, &c.mat, &c.mat2,
+                     &c.mat3, &c.eval, &c.einfo, &c.evwork, &c.matT, &c.dots, &c.gemm_part, &c.opbuf,
+                     &c.cg.buf, &c.cg.flags, &c.nl.row_ptr, &c.nl.row_idx, &c.nl.row_val64,
+                     &c.nl.row_val32, &c.nl.col_ptr, &c.nl.col_lma, &c.nl.col_val64,
+                     &c.nl.col_val32, &c.nl.scale64, &c.nl.d_ptr, &c.nl.d_idx, &c.nl.d_val,
+                     &c.nl.amap_dev};
+    for (DevMem *m : all) m->release();
+    if (c.blas) rocblas_destroy_handle(c.blas);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c.blas = nullptr;
+    c.stream = nullptr;
+    c.init = false;
+    c.host_unk = nullptr;
+    c.MB = 0;
+    c.wf_prec = -1;
+    c.prec = 0;
+    c.nl.nz = 0;
+    c.nl.natom = 0;
+    for (double &x : c.timers.ms) x = 0.0;
+    c.hpsi_bands = 0;
+    return kOK;
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+// Internal definitions of the MI355X Kohn-Sham library (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace rsdft {
+
+constexpr int kMaxMd = 8;
+
+enum Status {
+    kOK = 0,
+    kErrNotInit = 1,
+    kErrArg = 2,
+    kErrHip = 3,
+    kErrBlas = 4,
+    kErrSolver = 5,
+    kErrState = 6,
+};
+
+int set_error(int code, const char *fmt, ...);
+const char *last_error();
+
+#define RS_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::rsdft::set_error(::rsdft::kErrHip, "%s:%d %s -> %s", __FILE__,    \
+                                      __LINE__, #call, hipGetErrorString(e_));         \
+    } while (0)
+
+#define RS_BLAS(call)                                                                  \
+    do {                                                                               \
+        rocblas_status s_ = (call);                                                    \
+        if (s_ != rocblas_status_success)                                              \
+            return ::rsdft::set_error(::rsdft::kErrBlas, "%s:%d %s -> rocblas %d",     \
+                                      __FILE__, __LINE__, #call, (int)s_);             \
+    } while (0)
+
+#define RS_TRY(call)                                                                   \
+    do {                                                                               \
+        int r_ = (call);                                                               \
+        if (r_ != 0) return r_;                                                        \
+    } while (0)
+
+// Device allocation that frees itself.
+struct DevMem {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+    ~DevMem() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int ensure(size_t nbytes);   // grow-only
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Grid + FD description (periodic orthorhombic cell, Gamma point).
+struct GridDesc {
+    int N1 = 0, N2 = 0, N3 = 0, Md = 0;
+    long ML = 0;
+    double h[3] = {0, 0, 0};
+    double dV = 0.0;
+    double coef_lap0 = 0.0;
+    double coef_lap[3][kMaxMd] = {};   // [axis][m-1]
+};
+
+// Sparse projector operator in two orientations.
+//   rows  : projector lma -> its grid points (CSR over lma), used to project
+//   cols  : grid point -> (lma, value) (CSR over grid points), used to apply
+struct NonlocalTables {
+    int nz = 0;
+    long nnz = 0;
+    DevMem row_ptr, row_idx, row_val64, row_val32;
+    DevMem col_ptr, col_lma, col_val64, col_val32;
+    DevMem scale64;   // iuV(lma) * dV
+    // force tables
+    int natom = 0;
+    long nnzd = 0;
+    DevMem d_ptr, d_idx, d_val;   // CSR over lma of MJJ_MAP points, 3 values each (FP64)
+    std::vector<int> amap;        // 0-based atom of each lma (host)
+    DevMem amap_dev;
+};
+
+// Stencil launch geometry chosen for the grid.
+struct StencilGeom {
+    int px = 4;   // consecutive x points per thread
+    int ty = 1;   // y rows per workgroup
+    int tz = 1;   // z planes per workgroup (z-march length)
+    int threads = 64;
+    int tiles = 1;  // (N2/ty)*(N3/tz)
+};
+
+// CG per-band scalars (FP64 on device).
+struct CGScalars {
+    DevMem buf;       // all double arrays below, MB each
+    DevMem flags;     // int arrays
+    double *E, *E1, *rb, *res, *gkgk, *bk, *u1, *u2, *W, *rr0, *rr1, *pa, *pb;
+    int *active, *accept, *pc_active;
+    int ensure(int MB);
+};
+
+struct Timers {
+    double ms[8] = {};
+};
+
+struct Context {
+    bool init = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    GridDesc g;
+    StencilGeom geom_h;    // Md-order hamiltonian stencil
+    StencilGeom geom_pc;   // 7-point preconditioner stencil
+    int prec = 0;
+    int ncg = 3, mloop = 3;
+
+    DevMem vloc64, vloc32;
+    NonlocalTables nl;
+
+    // wavefunction mirror and work arrays, element type by precision
+    double *host_unk = nullptr;
+    int MB = 0;
+    int wf_prec = -1;
+    DevMem psi, hx, hp, gk, pgk, pk, rk, ppc, qpc, tmp;
+    DevMem coefC;      // projection coefficients [band][lma]
+    DevMem part;       // reduction partial sums
+    DevMem stage;      // host<->device staging (FP64)
+    DevMem opbuf;      // working-precision staging of the op_* entry points
+    DevMem mat, mat2, mat3, eval, einfo, evwork;   // nb x nb FP64 dense work
+    DevMem matT;       // nb x nb in working precision
+    DevMem dots;       // generic per-band result buffer (FP64)
+    CGScalars cg;
+    Timers timers;
+    long hpsi_bands = 0;
+    // BF16 split-GEMM scratch
+    DevMem gemm_part;
+};
+
+Context &ctx();
+
+// ---- launchers (implemented in the .hip files) ----------------------------
+// geometry
+StencilGeom choose_geom(int N1, int N2, int N3, int md, int px, int elem_bytes);
+
+// Hamiltonian pieces. flags bit0 kinetic, bit1 local, bit2 nonlocal, bit3 accumulate.
+enum { kKin = 1, kLoc = 2, kNl = 4, kAcc = 8 };
+template <class T>
+int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active = nullptr);
+template <class T>
+int nl_project(const T *in, int nb, T *coef, const int *band_active = nullptr);
+
+// preconditioner (precond_hprsdft) for all active bands: in g, in/out pg.
+template <class T>
+int precond_batched(const T *g, T *pg, int nb, const int *active);
+
+// reductions: out[k*nb + b] = scale * sum_i a_k[b,i]*b_k[b,i], k < ndot
+template <class T>
+int band_dots(int ndot, const T *const *a, const T *const *b, int nb, double scale,
+              double *out, const int *active = nullptr);
+
+// CG driver (batched conjugate_gradient_1)
+template <class T>
+int cg_batched(T *x, int nb, double *esp_dev, double *res_dev);
+
+// dense pieces
+template <class T>
+int gram_schmidt_dev(T *x, int nb);
+template <class T>
+int subspace_diag_dev(T *x, int nb, double *esp_dev);
+
+// adjoint
+template <class T>
+int density_dev(const T *x, int nb, const double *occ_dev, double *rho_dev);
+template <class T>
+int force_dev(const T *x, int nb, const double *occ_dev, double *force_dev_out);
+
+// element-wise helpers
+template <class T>
+int convert_from_f64(const double *src, T *dst, long n);
+template <class T>
+int convert_to_f64(const T *src, double *dst, long n);
+
+}  // namespace rsdft

@@ -1,0 +1,446 @@
+// Hamiltonian apply for gfx950: FD kinetic stencil + local potential +
+// Kleinman-Bylander nonlocal projectors, and the 7-point preconditioner
+// operator.
+//
+// Stencil design (one band per workgroup column):
+//   * a workgroup owns `ty` full x-rows of the (x,y) plane and marches `tz`
+//     planes in z; every thread owns `PX` consecutive x points of one row;
+//   * the 2*Md+1 z-neighbours of a thread's points live in a register queue
+//     (one new plane loaded per step), so z costs no LDS traffic;
+//   * the current plane (own rows + Md-deep y halo, full x so the x wrap is
+//     a modular index into LDS) is staged in LDS for the x and y terms;
+//   * periodic wrap in y/z is modular indexing of the global loads: the
+//     reference's bcset_3 halo exchange (bcset_3_module.f90:15-151,
+//     bc_module.f90:722-736) has no counterpart;
+//   * the epilogue adds Vloc*psi (op_localpot) and pulls the nonlocal term
+//     from a point-major CSR of the projector matrix, so H psi is written
+//     once.
+#include "common.h"
+
+namespace rsdft {
+
+template <class T, int MD>
+struct StencilCoef {
+    T c0;
+    T cx[MD], cy[MD], cz[MD];
+};
+
+__device__ __forceinline__ int wrapi(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+// XCD-aware work mapping: workgroup L runs on XCD L % 8; give each XCD a
+// contiguous range of work items so neighbouring items share its L2.
+__device__ __forceinline__ long xcd_work_index(long L, long total) {
+    if ((total & 7) != 0) return L;
+    const long per = total >> 3;
+    return (L & 7) * per + (L >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Md-order hamiltonian stencil with fused epilogue
+// ---------------------------------------------------------------------------
+template <class T, int MD, int PX>
+__global__ void __launch_bounds__(1024)
+k_hpsi(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, int N3,
+       int ty, int tz, int tpr, StencilCoef<T, MD> cf, const T *__restrict__ vloc,
+       const int *__restrict__ cptr, const int *__restrict__ clma,
+       const T *__restrict__ cval, const T *__restrict__ coefC, int nz, int flags,
+       const int *__restrict__ band_active)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T *L = reinterpret_cast<T *>(smem_raw);
+    const int band = blockIdx.y;
+    if (band_active && !band_active[band]) return;
+    const int nty = N2 / ty;
+    const int y0 = (blockIdx.x % nty) * ty;
+    const int z0 = (blockIdx.x / nty) * tz;
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int r = tid / tpr;
+    const int xs = (tid % tpr) * PX;
+    const bool own = r < ty;
+    const int yrow = y0 + (own ? r : 0);
+    const long plane = (long)N1 * N2;
+    const T *src = in + (long)band * ML;
+    T *dst = out + (long)band * ML;
+    const bool kin = flags & kKin, loc = flags & kLoc, nl = flags & kNl, acc_out = flags & kAcc;
+
+    T q[2 * MD + 1][PX];
+#pragma unroll
+    for (int k = 0; k < 2 * MD; ++k) {
+        const int zz = wrapi(z0 - MD + k, N3);
+        const T *p = src + zz * plane + (long)yrow * N1 + xs;
+#pragma unroll
+        for (int px = 0; px < PX; ++px) q[k][px] = p[px];
+    }
+    T *Lrow = L + (r + MD) * N1;
+    const int nhalo = 2 * MD * tpr;
+    for (int z = z0; z < z0 + tz; ++z) {
+        {
+            int zn = z + MD;
+            if (zn >= N3) zn -= N3;
+            const T *p = src + zn * plane + (long)yrow * N1 + xs;
+#pragma unroll
+            for (int px = 0; px < PX; ++px) q[2 * MD][px] = p[px];
+        }
+        if (own) {
+#pragma unroll
+            for (int px = 0; px < PX; ++px) Lrow[xs + px] = q[MD][px];
+        }
+        {
+            const T *zp = src + (long)z * plane;
+            for (int hr = tid; hr < nhalo; hr += nthr) {
+                const int hh = hr / tpr;
+                const int seg = (hr - hh * tpr) * PX;
+                int yy, lrow;
+                if (hh < MD) {
+                    yy = y0 - MD + hh;
+                    lrow = hh;
+                } else {
+                    yy = y0 + ty + hh - MD;
+                    lrow = ty + hh;
+                }
+                yy = wrapi(yy, N2);
+                const T *p = zp + (long)yy * N1 + seg;
+                T *d = L + lrow * N1 + seg;
+#pragma unroll
+                for (int px = 0; px < PX; ++px) d[px] = p[px];
+            }
+        }
+        __syncthreads();
+        if (own) {
+#pragma unroll
+            for (int px = 0; px < PX; ++px) {
+                const int x = xs + px;
+                T acc = T(0);
+                if (kin) {
+                    acc = cf.c0 * q[MD][px];
+#pragma unroll
+                    for (int m = 1; m <= MD; ++m) {
+                        int xm = x - m;
+                        if (xm < 0) xm += N1;
+                        int xp = x + m;
+                        if (xp >= N1) xp -= N1;
+                        acc = acc + cf.cx[m - 1] * (Lrow[xm] + Lrow[xp])
+                                  + cf.cy[m - 1] * (Lrow[xs + px - m * N1] + Lrow[xs + px + m * N1])
+                                  + cf.cz[m - 1] * (q[MD - m][px] + q[MD + m][px]);
+                    }
+                }
+                const long gi = (long)z * plane + (long)yrow * N1 + x;
+                if (loc) acc = acc + vloc[gi] * q[MD][px];
+                if (nl) {
+                    const int e1 = cptr[gi + 1];
+                    const T *cb = coefC + (long)band * nz;
+                    for (int e = cptr[gi]; e < e1; ++e) acc = acc + cval[e] * cb[clma[e]];
+                }
+                if (acc_out) acc = dst[gi] + acc;
+                dst[gi] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2 * MD; ++k) {
+#pragma unroll
+            for (int px = 0; px < PX; ++px) q[k][px] = q[k + 1][px];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Nonlocal projection: coefC[b][lma] = iuV(lma)*dV*sum_j uV(j,lma) psi_b(JJP(j,lma))
+// (ps_nloc2_op_module.f90:161-175), FP64 accumulation, NBB bands per block.
+// ---------------------------------------------------------------------------
+template <int NBB>
+__device__ __forceinline__ void block_reduce_store(double (&acc)[NBB], double *red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NBB; ++k) {
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[wave * NBB + k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NBB) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += red[w * NBB + threadIdx.x];
+        acc[0] = s;   // thread k holds the total of band k in acc[0]
+    }
+}
+
+template <class T, int NBB>
+__global__ void __launch_bounds__(256)
+k_project(const T *__restrict__ in, long ML, int nb, int nz, const int *__restrict__ rptr,
+          const int *__restrict__ ridx, const T *__restrict__ rval,
+          const double *__restrict__ scale, T *__restrict__ coefC,
+          const int *__restrict__ band_active)
+{
+    __shared__ double red[16 * NBB];
+    const int nbblk = (nb + NBB - 1) / NBB;
+    const long total = (long)nz * nbblk;
+    const long w = xcd_work_index(blockIdx.x, total);
+    const int lma = (int)(w % nz);
+    const int b0 = (int)(w / nz) * NBB;
+    double acc[NBB];
+#pragma unroll
+    for (int k = 0; k < NBB; ++k) acc[k] = 0.0;
+    bool act[NBB];
+#pragma unroll
+    for (int k = 0; k < NBB; ++k)
+        act[k] = (b0 + k < nb) && (!band_active || band_active[b0 + k]);
+    const int j1 = rptr[lma + 1];
+    for (int j = rptr[lma] + threadIdx.x; j < j1; j += blockDim.x) {
+        const int i = ridx[j];
+        const double v = (double)rval[j];
+#pragma unroll
+        for (int k = 0; k < NBB; ++k)
+            if (act[k]) acc[k] += v * (double)in[(long)(b0 + k) * ML + i];
+    }
+    block_reduce_store<NBB>(acc, red);
+    if (threadIdx.x < NBB) {
+        const int b = b0 + threadIdx.x;
+        if (b < nb && (!band_active || band_active[b]))
+            coefC[(long)b * nz + lma] = (T)(scale[lma] * acc[0]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 7-point preconditioner operator of precond_cg_mat2_omp with E = 0
+// (cgpc_hprsdft_module.f90:281-348):
+//   (A f)(i) = c f(i) + sum_d c_d (f(i+e_d) + f(i-e_d)),
+//   c = sum_d 1/h_d^2, c_d = -0.5/h_d^2.
+// mode 0:  f = g:   Q = A g; R = g - Q; PP = R; PG = g; part = sum R^2
+// mode 1:  f = PP:  Q = A PP;                    part = sum PP*Q
+// ---------------------------------------------------------------------------
+template <class T, int PX>
+__global__ void __launch_bounds__(1024)
+k_pc_stencil(const T *__restrict__ in, T *__restrict__ Q, T *__restrict__ R,
+             T *__restrict__ PP, T *__restrict__ PG, long ML, int N1, int N2, int N3, int ty, int tz, int tpr,
+             T c0, T c1, T c2, T c3, int mode, const int *__restrict__ band_active,
+             double *__restrict__ part, int tiles)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T *L = reinterpret_cast<T *>(smem_raw);
+    __shared__ double red[16];
+    const int band = blockIdx.y;
+    if (band_active && !band_active[band]) return;
+    const int nty = N2 / ty;
+    const int y0 = (blockIdx.x % nty) * ty;
+    const int z0 = (blockIdx.x / nty) * tz;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int r = tid / tpr;
+    const int xs = (tid % tpr) * PX;
+    const bool own = r < ty;
+    const int yrow = y0 + (own ? r : 0);
+    const long plane = (long)N1 * N2;
+    const long boff = (long)band * ML;
+    const T *src = in + boff;
+    T qm[PX], q0[PX], qp[PX];
+    {
+        const int zz = wrapi(z0 - 1, N3);
+        const T *p = src + zz * plane + (long)yrow * N1 + xs;
+        const T *p0 = src + (long)z0 * plane + (long)yrow * N1 + xs;
+#pragma unroll
+        for (int px = 0; px < PX; ++px) { qm[px] = p[px]; q0[px] = p0[px]; }
+    }
+    T *Lrow = L + (r + 1) * N1;
+    double sum = 0.0;
+    for (int z = z0; z < z0 + tz; ++z) {
+        {
+            int zn = z + 1;
+            if (zn >= N3) zn -= N3;
+            const T *p = src + zn * plane + (long)yrow * N1 + xs;
+#pragma unroll
+            for (int px = 0; px < PX; ++px) qp[px] = p[px];
+        }
+        if (own) {
+#pragma unroll
+            for (int px = 0; px < PX; ++px) Lrow[xs + px] = q0[px];
+        }
+        for (int hr = tid; hr < 2 * tpr; hr += nthr) {
+            const int hh = hr / tpr;
+            const int seg = (hr - hh * tpr) * PX;
+            const int yy = wrapi(hh == 0 ? y0 - 1 : y0 + ty, N2);
+            const int lrow = hh == 0 ? 0 : ty + 1;
+            const T *p = src + (long)z * plane + (long)yy * N1 + seg;
+#pragma unroll
+            for (int px = 0; px < PX; ++px) L[lrow * N1 + seg + px] = p[px];
+        }
+        __syncthreads();
+        if (own) {
+#pragma unroll
+            for (int px = 0; px < PX; ++px) {
+                const int x = xs + px;
+                int xm = x - 1;
+                if (xm < 0) xm += N1;
+                int xp = x + 1;
+                if (xp >= N1) xp -= N1;
+                T f = c0 * q0[px];
+                f = f + c1 * (Lrow[xp] + Lrow[xm]) + c2 * (Lrow[x + N1] + Lrow[x - N1])
+                      + c3 * (qp[px] + qm[px]);
+                const long gi = boff + (long)z * plane + (long)yrow * N1 + x;
+                if (mode == 0) {
+                    const T rr = q0[px] - f;
+                    R[gi] = rr;
+                    PP[gi] = rr;
+                    PG[gi] = q0[px];
+                    sum += (double)rr * (double)rr;
+                } else {
+                    Q[gi] = f;
+                    sum += (double)q0[px] * (double)f;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int px = 0; px < PX; ++px) { qm[px] = q0[px]; q0[px] = qp[px]; }
+    }
+    const int lane = tid & 63, wave = tid >> 6, nw = (nthr + 63) >> 6;
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
+    if (lane == 0) red[wave] = sum;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += red[w];
+        part[(long)band * tiles + blockIdx.x] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+StencilGeom choose_geom(int N1, int N2, int N3, int md, int px, int elem_bytes)
+{
+    StencilGeom g;
+    g.px = px;
+    const int tpr = N1 / px;
+    double best = -1.0;
+    for (int ty = 1; ty <= N2; ++ty) {
+        if (N2 % ty) continue;
+        const int thr = tpr * ty;
+        if (thr > 512) break;
+        const size_t lds = (size_t)(ty + 2 * md) * N1 * elem_bytes;
+        if (lds > 48 * 1024) continue;
+        const int waves = (thr + 63) / 64;
+        const double eff = (double)thr / (64.0 * waves);
+        // favour fuller waves, then taller tiles (less halo), then >= 128 threads
+        const double score = eff * 4.0 + (double)ty / (ty + 2.0 * md) + (thr >= 128 ? 1.0 : 0.0);
+        if (score > best) {
+            best = score;
+            g.ty = ty;
+            g.threads = thr;
+        }
+    }
+    // z-march length: long enough to amortise the 2*Md queue warm-up
+    int tz = N3;
+    for (int c = 1; c <= N3; ++c) {
+        if (N3 % c) continue;
+        if (N3 / c >= 4 * md + 8 || c == 1) tz = N3 / c;
+        if (N3 / c < 4 * md + 8) break;
+    }
+    if (tz > 48) {
+        for (int c = 2; c <= N3; ++c)
+            if (N3 % c == 0 && N3 / c <= 48 && N3 / c >= 2 * md + 4) { tz = N3 / c; break; }
+    }
+    g.tz = tz;
+    g.tiles = (N2 / g.ty) * (N3 / g.tz);
+    return g;
+}
+
+template <class T, int MD>
+static int launch_hpsi_md(const T *in, T *out, int nb, int flags, const int *band_active,
+                          const T *vloc, const T *cval, const T *coefC)
+{
+    Context &c = ctx();
+    const GridDesc &g = c.g;
+    const StencilGeom &G = c.geom_h;
+    StencilCoef<T, MD> cf;
+    cf.c0 = (T)g.coef_lap0;
+    for (int m = 0; m < MD; ++m) {
+        cf.cx[m] = (T)g.coef_lap[0][m];
+        cf.cy[m] = (T)g.coef_lap[1][m];
+        cf.cz[m] = (T)g.coef_lap[2][m];
+    }
+    const int tpr = g.N1 / G.px;
+    const size_t lds = (size_t)(G.ty + 2 * MD) * g.N1 * sizeof(T);
+    dim3 grid(G.tiles, nb);
+    hipLaunchKernelGGL((k_hpsi<T, MD, 4>), grid, dim3(G.threads), lds, c.stream, in, out, g.ML,
+                       g.N1, g.N2, g.N3, G.ty, G.tz, tpr, cf, vloc, c.nl.col_ptr.as<int>(),
+                       c.nl.col_lma.as<int>(), cval, coefC, c.nl.nz, flags, band_active);
+    RS_HIP(hipGetLastError());
+    return kOK;
+}
+
+template <class T>
+int nl_project(const T *in, int nb, T *coef, const int *band_active)
+{
+    Context &c = ctx();
+    if (c.nl.nz <= 0) return kOK;
+    constexpr int NBB = 8;
+    const long nblk = (long)c.nl.nz * ((nb + NBB - 1) / NBB);
+    const T *val = sizeof(T) == 8 ? (const T *)c.nl.row_val64.p : (const T *)c.nl.row_val32.p;
+    hipLaunchKernelGGL((k_project<T, NBB>), dim3(nblk), dim3(256), 0, c.stream, in, c.g.ML, nb,
+                       c.nl.nz, c.nl.row_ptr.as<int>(), c.nl.row_idx.as<int>(), val,
+                       c.nl.scale64.as<double>(), coef, band_active);
+    RS_HIP(hipGetLastError());
+    return kOK;
+}
+
+template <class T>
+int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
+{
+    Context &c = ctx();
+    if (nb <= 0) return kOK;
+    const bool have_nl = c.nl.nz > 0;
+    if (!have_nl) flags &= ~kNl;
+    const T *coefC = nullptr;
+    if (flags & kNl) {
+        RS_TRY(c.coefC.ensure((size_t)nb * c.nl.nz * sizeof(T)));
+        RS_TRY(nl_project<T>(in, nb, c.coefC.as<T>(), band_active));
+        coefC = c.coefC.as<T>();
+    }
+    const T *vloc = sizeof(T) == 8 ? (const T *)c.vloc64.p : (const T *)c.vloc32.p;
+    const T *cval = sizeof(T) == 8 ? (const T *)c.nl.col_val64.p : (const T *)c.nl.col_val32.p;
+    if ((flags & kLoc) && !vloc) return set_error(kErrState, "local potential not set");
+    c.hpsi_bands += nb;
+    switch (c.g.Md) {
+    case 1: return launch_hpsi_md<T, 1>(in, out, nb, flags, band_active, vloc, cval, coefC);
+    case 2: return launch_hpsi_md<T, 2>(in, out, nb, flags, band_active, vloc, cval, coefC);
+    case 3: return launch_hpsi_md<T, 3>(in, out, nb, flags, band_active, vloc, cval, coefC);
+    case 4: return launch_hpsi_md<T, 4>(in, out, nb, flags, band_active, vloc, cval, coefC);
+    case 6: return launch_hpsi_md<T, 6>(in, out, nb, flags, band_active, vloc, cval, coefC);
+    case 8: return launch_hpsi_md<T, 8>(in, out, nb, flags, band_active, vloc, cval, coefC);
+    default: return set_error(kErrArg, "Md=%d not compiled (1,2,3,4,6,8)", c.g.Md);
+    }
+}
+
+template <class T>
+int pc_stencil(const T *in, T *Q, T *R, T *PP, T *PG, int nb, int mode, const int *active,
+               double *part)
+{
+    Context &c = ctx();
+    const GridDesc &g = c.g;
+    const StencilGeom &G = c.geom_pc;
+    const double hx = 1.0 / (g.h[0] * g.h[0]), hy = 1.0 / (g.h[1] * g.h[1]),
+                 hz = 1.0 / (g.h[2] * g.h[2]);
+    const T c0 = (T)(hx + hy + hz), c1 = (T)(-0.5 * hx), c2 = (T)(-0.5 * hy), c3 = (T)(-0.5 * hz);
+    const int tpr = g.N1 / G.px;
+    const size_t lds = (size_t)(G.ty + 2) * g.N1 * sizeof(T);
+    hipLaunchKernelGGL((k_pc_stencil<T, 4>), dim3(G.tiles, nb), dim3(G.threads), lds, c.stream,
+                       in, Q, R, PP, PG, g.ML, g.N1, g.N2, g.N3, G.ty, G.tz, tpr, c0, c1, c2, c3, mode,
+                       active, part, G.tiles);
+    RS_HIP(hipGetLastError());
+    return kOK;
+}
+
+template int hpsi_apply<double>(const double *, double *, int, int, const int *);
+template int hpsi_apply<float>(const float *, float *, int, int, const int *);
+template int nl_project<double>(const double *, int, double *, const int *);
+template int nl_project<float>(const float *, int, float *, const int *);
+template int pc_stencil<double>(const double *, double *, double *, double *, double *, int,
+                                int, const int *, double *);
+template int pc_stencil<float>(const float *, float *, float *, float *, float *, int, int,
+                               const int *, double *);
+
+}  // namespace rsdft

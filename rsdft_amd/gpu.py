@@ -1,0 +1,271 @@
+"""Python host mirror of the reference's operator interface over the C-ABI.
+
+``RsdftGpu`` loads ``librsdft_gpu.so`` (built in-tree by ``rsdft_amd.build``)
+and exposes the reference's routines with their argument meaning:
+``hamiltonian`` / ``op_kinetic`` / ``op_localpot`` / ``op_nonlocal``
+(hamiltonian_module.f90:20), ``conjugate_gradient`` (cg_module.f90:54),
+``gram_schmidt`` (gram_schmidt_module.f90:28), ``subspace_diag``
+(subspace_diag_module.f90:17), ``sweep`` (sweep_module.f90:147-158),
+``calc_density`` (density_module.f90:115) and ``calc_force_ps_nloc2``
+(ps_nloc2_module.f90:1281).  Arrays are numpy, band-major ``(nb, ML)``.
+A non-zero status from the library raises :class:`RsdftGpuError` carrying
+its message, as the reference's ``stop`` would end the run.
+
+There is no CPU fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librsdft_gpu.so")
+
+PREC_FP64 = 0
+PREC_MIXED_FP32 = 1
+PREC_MIXED_BF16 = 2
+
+# every C-ABI entry point of include/rsdft_gpu.h
+EXPORTS = (
+    "rsdft_gpu_init", "rsdft_gpu_set_vloc", "rsdft_gpu_set_projectors",
+    "rsdft_gpu_set_projector_derivatives", "rsdft_gpu_set_precision",
+    "rsdft_gpu_set_solver_params", "rsdft_gpu_hamiltonian", "rsdft_gpu_op_kinetic",
+    "rsdft_gpu_op_localpot", "rsdft_gpu_op_nonlocal", "rsdft_gpu_wf_attach",
+    "rsdft_gpu_wf_sync", "rsdft_gpu_wf_detach", "rsdft_gpu_conjugate_gradient",
+    "rsdft_gpu_gram_schmidt", "rsdft_gpu_subspace_diag", "rsdft_gpu_sweep",
+    "rsdft_gpu_calc_density", "rsdft_gpu_force_ps_nloc2", "rsdft_gpu_band_energy_grad",
+    "rsdft_gpu_test_hpsi", "rsdft_gpu_get_timers", "rsdft_gpu_reset_timers",
+    "rsdft_gpu_finalize", "rsdft_gpu_last_error",
+)
+
+
+class RsdftGpuError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RsdftGpuError(f"{path} is missing: run rsdft_amd.build.build() first")
+    lib = ctypes.CDLL(path)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    sig = {
+        "rsdft_gpu_init": [I, I, I, P, D, I, P, D],
+        "rsdft_gpu_set_vloc": [P, I],
+        "rsdft_gpu_set_projectors": [I, I, P, P, P, P],
+        "rsdft_gpu_set_projector_derivatives": [I, P, I, P, P, P],
+        "rsdft_gpu_set_precision": [I],
+        "rsdft_gpu_set_solver_params": [I, I],
+        "rsdft_gpu_hamiltonian": [I, I, P, P, I, I, I, I],
+        "rsdft_gpu_op_kinetic": [I, P, P, I, I, I, I],
+        "rsdft_gpu_op_localpot": [I, I, I, P, P],
+        "rsdft_gpu_op_nonlocal": [I, I, P, P, I, I, I, I],
+        "rsdft_gpu_wf_attach": [P, I, I],
+        "rsdft_gpu_wf_sync": [I],
+        "rsdft_gpu_wf_detach": [],
+        "rsdft_gpu_conjugate_gradient": [I, I, I, I, I, P, P, P],
+        "rsdft_gpu_gram_schmidt": [P, I, I, I, I],
+        "rsdft_gpu_subspace_diag": [P, P, I, I],
+        "rsdft_gpu_sweep": [P, P, P, I, I, I],
+        "rsdft_gpu_calc_density": [P, I, I, P, P],
+        "rsdft_gpu_force_ps_nloc2": [P, I, I, P, P],
+        "rsdft_gpu_band_energy_grad": [P, I, I, P, P, P, P],
+        "rsdft_gpu_test_hpsi": [I, P],
+        "rsdft_gpu_get_timers": [P, I],
+        "rsdft_gpu_reset_timers": [],
+        "rsdft_gpu_finalize": [],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = I
+    lib.rsdft_gpu_last_error.argtypes = []
+    lib.rsdft_gpu_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class RsdftGpu:
+    """One GPU context for one system (grid, Vloc, projector tables)."""
+
+    def __init__(self, system, precision: int = PREC_FP64, ncg: int = 3, mloop: int = 3,
+                 with_derivatives: bool = True, proj=None, vloc=None):
+        self.lib = load_library()
+        self.sys = system
+        g = system.grid
+        self.ML = g.ML
+        self.dV = g.dV
+        self._keep = []
+        coef = _f64(np.asarray(system.coef_lap).T.ravel())       # Fortran coef_lap(3, Md)
+        h = _f64(g.h)
+        self._call("rsdft_gpu_init", g.N[0], g.N[1], g.N[2], _p(h), g.dV, system.Md, _p(coef),
+                   system.coef_lap0)
+        self.set_vloc(system.vloc if vloc is None else vloc)
+        self.set_projectors(system.proj if proj is None else proj, with_derivatives)
+        self._call("rsdft_gpu_set_solver_params", ncg, mloop)
+        self.set_precision(precision)
+        self._attached = None
+
+    # -- plumbing ----------------------------------------------------------------
+    def _call(self, name, *args):
+        r = getattr(self.lib, name)(*args)
+        if r != 0:
+            msg = self.lib.rsdft_gpu_last_error().decode(errors="replace")
+            raise RsdftGpuError(f"{name} -> {r}: {msg}")
+        return r
+
+    def set_precision(self, mode: int):
+        self._call("rsdft_gpu_set_precision", int(mode))
+        self.precision = int(mode)
+
+    def set_vloc(self, vloc):
+        v = _f64(vloc)
+        self._call("rsdft_gpu_set_vloc", _p(v), 1)
+
+    def set_projectors(self, pr, with_derivatives=True):
+        self._tabs = (_i32(pr.MJJ), _i32(pr.JJP), _f64(pr.uVk), _i32(pr.iuV))
+        MJJ, JJP, uVk, iuV = self._tabs
+        self._call("rsdft_gpu_set_projectors", pr.nzlma, pr.MAXMJJ, _p(MJJ), _p(JJP), _p(uVk),
+                   _p(iuV))
+        if with_derivatives:
+            self._dtabs = (_i32(pr.amap), _i32(pr.MJJ_MAP), _i32(pr.JJ_MAP), _f64(pr.duVdR))
+            amap, mjm, jjm, dud = self._dtabs
+            natom = int(np.max(pr.amap))
+            self._call("rsdft_gpu_set_projector_derivatives", natom, _p(amap), pr.MMJJ, _p(mjm),
+                       _p(jjm), _p(dud))
+        self.natom = int(np.max(pr.amap))
+
+    # -- operators (hamiltonian_module.f90:20-72) --------------------------------
+    def hamiltonian(self, tpsi):
+        t = _f64(np.atleast_2d(tpsi))
+        h = np.empty_like(t)
+        self._call("rsdft_gpu_hamiltonian", 1, 1, _p(t), _p(h), 1, self.ML, 1, t.shape[0])
+        return h
+
+    def op_kinetic(self, tpsi):
+        t = _f64(np.atleast_2d(tpsi))
+        h = np.empty_like(t)
+        self._call("rsdft_gpu_op_kinetic", 1, _p(t), _p(h), 1, self.ML, 1, t.shape[0])
+        return h
+
+    def op_localpot(self, f, vf=None):
+        f = _f64(np.atleast_2d(f))
+        vf = np.zeros_like(f) if vf is None else _f64(vf).copy()
+        self._call("rsdft_gpu_op_localpot", 1, self.ML, f.shape[0], _p(f), _p(vf))
+        return vf
+
+    def op_nonlocal(self, tpsi, htpsi=None):
+        t = _f64(np.atleast_2d(tpsi))
+        h = np.zeros_like(t) if htpsi is None else _f64(htpsi).copy()
+        self._call("rsdft_gpu_op_nonlocal", 1, 1, _p(t), _p(h), 1, self.ML, 1, t.shape[0])
+        return h
+
+    # -- residency -----------------------------------------------------------------
+    def attach(self, unk: np.ndarray):
+        """Device mirror of unk (float64, C-contiguous (MB, ML)); kept by reference."""
+        if unk.dtype != np.float64 or not unk.flags.c_contiguous:
+            raise ValueError("attach needs a C-contiguous float64 array")
+        self._call("rsdft_gpu_wf_attach", _p(unk), self.ML, unk.shape[0])
+        self._attached = unk
+
+    def sync(self, to_host: bool = True):
+        self._call("rsdft_gpu_wf_sync", 0 if to_host else 1)
+
+    def detach(self):
+        self._call("rsdft_gpu_wf_detach")
+        self._attached = None
+
+    def _wf(self, unk):
+        if self._attached is not None and unk is self._attached:
+            return unk
+        return _f64(unk).copy()
+
+    # -- eigensolver -----------------------------------------------------------------
+    def conjugate_gradient(self, unk):
+        x = self._wf(unk)
+        MB = x.shape[0]
+        esp, res = np.zeros(MB), np.zeros(MB)
+        self._call("rsdft_gpu_conjugate_gradient", 1, self.ML, MB, 1, 1, _p(x), _p(esp), _p(res))
+        return x, esp, res
+
+    def gram_schmidt(self, unk):
+        x = self._wf(unk)
+        self._call("rsdft_gpu_gram_schmidt", _p(x), self.ML, x.shape[0], 1, x.shape[0])
+        return x
+
+    def subspace_diag(self, unk):
+        x = self._wf(unk)
+        esp = np.zeros(x.shape[0])
+        self._call("rsdft_gpu_subspace_diag", _p(x), _p(esp), self.ML, x.shape[0])
+        return x, esp
+
+    def sweep(self, unk, nsweep: int = 1):
+        x = self._wf(unk)
+        esp, res = np.zeros(x.shape[0]), np.zeros(x.shape[0])
+        self._call("rsdft_gpu_sweep", _p(x), _p(esp), _p(res), self.ML, x.shape[0], nsweep)
+        return x, esp, res
+
+    # -- adjoints ----------------------------------------------------------------------
+    def calc_density(self, unk, occ):
+        x = self._wf(unk)
+        occ = _f64(occ)
+        rho = np.zeros(self.ML)
+        self._call("rsdft_gpu_calc_density", _p(x), self.ML, x.shape[0], _p(occ), _p(rho))
+        return rho
+
+    def calc_force_ps_nloc2(self, unk, occ):
+        x = self._wf(unk)
+        occ = _f64(occ)
+        f = np.zeros(3 * self.natom)
+        self._call("rsdft_gpu_force_ps_nloc2", _p(x), self.ML, x.shape[0], _p(occ), _p(f))
+        return f.reshape(self.natom, 3)
+
+    def band_energy_grad(self, unk, occ):
+        """(E, dE/dVloc (ML,), dE/dR (natom, 3)) at fixed psi."""
+        x = self._wf(unk)
+        occ = _f64(occ)
+        E = np.zeros(1)
+        dV = np.zeros(self.ML)
+        dR = np.zeros(3 * self.natom)
+        self._call("rsdft_gpu_band_energy_grad", _p(x), self.ML, x.shape[0], _p(occ), _p(E),
+                   _p(dV), _p(dR))
+        return float(E[0]), dV, dR.reshape(self.natom, 3)
+
+    # -- measurement ---------------------------------------------------------------------
+    def test_hpsi(self, nrep: int = 10):
+        """ms per full-band application: (hamiltonian, kinetic, localpot, nonlocal)."""
+        t = np.zeros(4)
+        self._call("rsdft_gpu_test_hpsi", nrep, _p(t))
+        return t
+
+    def timers(self):
+        t = np.zeros(8)
+        self._call("rsdft_gpu_get_timers", _p(t), 8)
+        return {"hamiltonian": t[0], "precond": t[1], "cg": t[2], "gram_schmidt": t[3],
+                "subspace_diag": t[4], "sweep": t[5], "hpsi_bands": t[6]}
+
+    def reset_timers(self):
+        self._call("rsdft_gpu_reset_timers")
+
+    def finalize(self):
+        self._call("rsdft_gpu_finalize")
