@@ -123,9 +123,11 @@ RSDFT_API int rsdft_gpu_force_ps_nloc2(const double *unk, int ML0, int MB, const
 RSDFT_API int rsdft_gpu_band_energy_grad(const double *unk, int ML0, int MB, const double *occ,
                                double *E, double *dEdV, double *dEdR);
 
-/* test_hpsi2 (test_hpsi2_module.f90:16-143): times nrep applications of
- * hamiltonian, op_kinetic, op_localpot, op_nonlocal to the attached
- * wavefunctions; times_ms[4] = mean milliseconds per full-band application. */
+/* test_hpsi2 (test_hpsi2_module.f90:16-143): times nrep applications to the
+ * attached wavefunctions on the library's stream; times_ms[6] = mean ms per
+ * all-band application of [0] hamiltonian, [1] op_kinetic, [2] op_localpot,
+ * [3] op_nonlocal, [4] the fused stencil kernel alone, [5] the projection
+ * kernel alone. */
 RSDFT_API int rsdft_gpu_test_hpsi(int nrep, double *times_ms);
 
 /* Per-phase timers in milliseconds since the last reset (watch_module.f90

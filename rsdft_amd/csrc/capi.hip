@@ -622,16 +622,26 @@ int rsdft_gpu_test_hpsi(int nrep, double *times_ms)
     RS_TRY(need_init());
     Context &c = ctx();
     if (!c.host_unk) return set_error(kErrState, "attach wavefunctions first");
-    const int flags[4] = {kKin | kLoc | kNl, kKin, kLoc | kAcc, kNl | kAcc};
+    // [0] hamiltonian, [1] op_kinetic, [2] op_localpot, [3] op_nonlocal,
+    // [4] fused stencil kernel alone (projection coefficients reused),
+    // [5] projection kernel alone
+    const int flags[6] = {kKin | kLoc | kNl, kKin, kLoc | kAcc, kNl | kAcc, -1, -2};
     hipEvent_t a, b;
     RS_HIP(hipEventCreate(&a));
     RS_HIP(hipEventCreate(&b));
-    for (int t = 0; t < 4; ++t) {
-        // one untimed application, then nrep timed
+    const long saved = c.hpsi_bands;
+    for (int t = 0; t < 6; ++t) {
         for (int rep = -1; rep < nrep; ++rep) {
             if (rep == 0) RS_HIP(hipEventRecord(a, c.stream));
-            if (c.prec == 0) RS_TRY(hpsi_apply<double>(psiT<double>(), c.hp.as<double>(), c.MB, flags[t]));
-            else RS_TRY(hpsi_apply<float>(psiT<float>(), c.hp.as<float>(), c.MB, flags[t]));
+            if (c.prec == 0) {
+                if (flags[t] >= 0) RS_TRY(hpsi_apply<double>(psiT<double>(), c.hp.as<double>(), c.MB, flags[t]));
+                else if (flags[t] == -1) RS_TRY(hpsi_stencil_only<double>(psiT<double>(), c.hp.as<double>(), c.MB));
+                else RS_TRY(nl_project<double>(psiT<double>(), c.MB, c.coefC.as<double>()));
+            } else {
+                if (flags[t] >= 0) RS_TRY(hpsi_apply<float>(psiT<float>(), c.hp.as<float>(), c.MB, flags[t]));
+                else if (flags[t] == -1) RS_TRY(hpsi_stencil_only<float>(psiT<float>(), c.hp.as<float>(), c.MB));
+                else RS_TRY(nl_project<float>(psiT<float>(), c.MB, c.coefC.as<float>()));
+            }
         }
         RS_HIP(hipEventRecord(b, c.stream));
         RS_HIP(hipEventSynchronize(b));
@@ -639,6 +649,7 @@ int rsdft_gpu_test_hpsi(int nrep, double *times_ms)
         RS_HIP(hipEventElapsedTime(&ms, a, b));
         if (times_ms) times_ms[t] = nrep > 0 ? ms / nrep : 0.0;
     }
+    c.hpsi_bands = saved;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     return kOK;

@@ -159,6 +159,9 @@ template <class T>
 int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active = nullptr);
 template <class T>
 int nl_project(const T *in, int nb, T *coef, const int *band_active = nullptr);
+// the fused stencil kernel alone, reusing the projection coefficients in place
+template <class T>
+int hpsi_stencil_only(const T *in, T *out, int nb);
 
 // preconditioner (precond_hprsdft) for all active bands: in g, in/out pg.
 template <class T>

@@ -416,6 +416,17 @@ int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
 }
 
 template <class T>
+int hpsi_stencil_only(const T *in, T *out, int nb)
+{
+    Context &c = ctx();
+    const T *vloc = sizeof(T) == 8 ? (const T *)c.vloc64.p : (const T *)c.vloc32.p;
+    const T *cval = sizeof(T) == 8 ? (const T *)c.nl.col_val64.p : (const T *)c.nl.col_val32.p;
+    const int flags = kKin | kLoc | (c.nl.nz > 0 ? kNl : 0);
+    if (c.g.Md != 6) return set_error(kErrArg, "stencil-only timing is built for Md=6");
+    return launch_hpsi_md<T, 6>(in, out, nb, flags, nullptr, vloc, cval, c.coefC.as<T>());
+}
+
+template <class T>
 int pc_stencil(const T *in, T *Q, T *R, T *PP, T *PG, int nb, int mode, const int *active,
                double *part)
 {
@@ -436,6 +447,8 @@ int pc_stencil(const T *in, T *Q, T *R, T *PP, T *PG, int nb, int mode, const in
 
 template int hpsi_apply<double>(const double *, double *, int, int, const int *);
 template int hpsi_apply<float>(const float *, float *, int, int, const int *);
+template int hpsi_stencil_only<double>(const double *, double *, int);
+template int hpsi_stencil_only<float>(const float *, float *, int);
 template int nl_project<double>(const double *, int, double *, const int *);
 template int nl_project<float>(const float *, int, float *, const int *);
 template int pc_stencil<double>(const double *, double *, double *, double *, double *, int,

@@ -253,8 +253,9 @@ class RsdftGpu:
 
     # -- measurement ---------------------------------------------------------------------
     def test_hpsi(self, nrep: int = 10):
-        """ms per full-band application: (hamiltonian, kinetic, localpot, nonlocal)."""
-        t = np.zeros(4)
+        """ms per all-band application: hamiltonian, kinetic, localpot, nonlocal,
+        fused stencil kernel alone, projection kernel alone."""
+        t = np.zeros(6)
         self._call("rsdft_gpu_test_hpsi", nrep, _p(t))
         return t
 
