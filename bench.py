@@ -35,13 +35,10 @@ def log(msg):
 
 
 def hpsi_stencil_bytes(sysm, nb, s):
-    """Algorithmic HBM bytes of one all-band launch of the fused stencil kernel:
-    psi read + H psi write per band, Vloc, the point-major projector table and
-    the projection coefficients once."""
+    """Algorithmic HBM bytes of one all-band launch of the stencil kernel
+    (kinetic + local): psi read + H psi write per band and Vloc once."""
     ML = sysm.grid.ML
-    nnz = int(np.sum(sysm.proj.MJJ))
-    nz = sysm.proj.nzlma
-    return 2 * ML * nb * s + ML * s + 4 * (ML + 1) + nnz * (4 + s) + nb * nz * s
+    return 2 * ML * nb * s + ML * s
 
 
 def cpu_baseline(sysm, x0, nsample):
@@ -191,7 +188,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_hpsi<float,6,4> (FD stencil + Vloc + nonlocal pull, all 480 bands)",
+            "kernel": "k_hpsi<float,6,4,2> (Md=6 FD stencil + Vloc, all 480 bands)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

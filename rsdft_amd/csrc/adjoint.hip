@@ -75,7 +75,7 @@ k_dproject(const T *__restrict__ X, long ML, int nb, int nz, const int *__restri
     }
 }
 
-// force2(d, a) = sum_b sum_{lma of a} (-2 f_b dV) C[b][lma] W[b][lma][d]
+// force2(d, a) = sum_b sum_{lma of a} (-2 f_b dV) C[lma][b] W[b][lma][d]
 // C already carries iuV*dV, so -2 f dV^2 iuV sum uV psi = -2 f dV C.
 template <class T>
 __global__ void __launch_bounds__(256)
@@ -93,7 +93,7 @@ k_force_reduce(const T *__restrict__ C, const double *__restrict__ W, int nb, in
         const int lma = alma[l0 + (int)(t - (long)b * nl)];
         const double f = occ[b];
         if (f == 0.0) continue;
-        const double w0 = -2.0 * f * dV * (double)C[(long)b * nz + lma];
+        const double w0 = -2.0 * f * dV * (double)C[(long)lma * nb + b];
         const double *w = W + ((long)b * nz + lma) * 3;
         s0 += w0 * w[0];
         s1 += w0 * w[1];

@@ -224,8 +224,11 @@ int rsdft_gpu_init(int N1, int N2, int N3, const double *Hgrid, double dV, int M
     c.g.coef_lap0 = coef_lap0;
     for (int m = 0; m < Md; ++m)
         for (int d = 0; d < 3; ++d) c.g.coef_lap[d][m] = coef_lap[d + 3 * m];
-    c.geom_h = choose_geom(N1, N2, N3, Md, 4, 8);
+    c.geom_h = choose_geom(N1, N2, N3, Md, 2, 8);
+    c.geom_h32 = choose_geom(N1, N2, N3, Md, 4, 4);
     c.geom_pc = choose_geom(N1, N2, N3, 1, 4, 8);
+    if (c.geom_h.threads == 0 || c.geom_h32.threads == 0 || c.geom_pc.threads == 0)
+        return set_error(kErrArg, "no stencil tiling for grid %dx%dx%d", N1, N2, N3);
     c.host_unk = nullptr;
     c.MB = 0;
     c.init = true;
@@ -284,6 +287,17 @@ int rsdft_gpu_set_projectors(int nzlma, int maxmjj, const int *MJJ, const int *J
             clma[e] = l;
             cval[e] = rval[rptr[l] + j];
         }
+    // covered points (inside some sphere) and their entry ranges
+    std::vector<int> cpts, cvptr;
+    for (long i = 0; i < ML; ++i)
+        if (cptr[i + 1] > cptr[i]) { cpts.push_back((int)i); cvptr.push_back(cptr[i]); }
+    cvptr.push_back((int)nnz);
+    const int ncov = (int)cpts.size();
+    int maxblk = 0;
+    for (int p0 = 0; p0 < ncov; p0 += 256) {
+        const int p1 = std::min(ncov, p0 + 256);
+        maxblk = std::max(maxblk, cvptr[p1] - cvptr[p0]);
+    }
     std::vector<float> rval32(rval.begin(), rval.end()), cval32(cval.begin(), cval.end());
     std::vector<double> scale(nzlma);
     for (int l = 0; l < nzlma; ++l) scale[l] = (double)iuV[l] * c.g.dV;
@@ -306,6 +320,14 @@ int rsdft_gpu_set_projectors(int nzlma, int maxmjj, const int *MJJ, const int *J
     RS_HIP(hipMemcpy(nl.col_val64.p, cval.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
     RS_HIP(hipMemcpy(nl.col_val32.p, cval32.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
     RS_HIP(hipMemcpy(nl.scale64.p, scale.data(), sizeof(double) * nzlma, hipMemcpyHostToDevice));
+    nl.ncov = ncov;
+    nl.max_blk_entries = maxblk;
+    if ((size_t)maxblk * (sizeof(int) + sizeof(double)) + 16 > 64 * 1024)
+        return set_error(kErrArg, "projector spheres too dense: %d entries per 256 points", maxblk);
+    RS_TRY(nl.cov_pts.ensure(sizeof(int) * std::max(ncov, 1)));
+    RS_TRY(nl.cov_ptr.ensure(sizeof(int) * (ncov + 1)));
+    if (ncov) RS_HIP(hipMemcpy(nl.cov_pts.p, cpts.data(), sizeof(int) * ncov, hipMemcpyHostToDevice));
+    RS_HIP(hipMemcpy(nl.cov_ptr.p, cvptr.data(), sizeof(int) * (ncov + 1), hipMemcpyHostToDevice));
     return kOK;
 }
 
@@ -623,24 +645,29 @@ int rsdft_gpu_test_hpsi(int nrep, double *times_ms)
     Context &c = ctx();
     if (!c.host_unk) return set_error(kErrState, "attach wavefunctions first");
     // [0] hamiltonian, [1] op_kinetic, [2] op_localpot, [3] op_nonlocal,
-    // [4] fused stencil kernel alone (projection coefficients reused),
-    // [5] projection kernel alone
-    const int flags[6] = {kKin | kLoc | kNl, kKin, kLoc | kAcc, kNl | kAcc, -1, -2};
+    // [4] stencil kernel (kinetic + local), [5] projection kernel,
+    // [6] nonlocal apply kernel
+    const int flags[7] = {kKin | kLoc | kNl, kKin, kLoc | kAcc, kNl | kAcc, -1, -2, -3};
     hipEvent_t a, b;
     RS_HIP(hipEventCreate(&a));
     RS_HIP(hipEventCreate(&b));
     const long saved = c.hpsi_bands;
-    for (int t = 0; t < 6; ++t) {
+    RS_TRY(c.coefC.ensure((size_t)c.MB * std::max(c.nl.nz, 1) * elem_bytes(c.prec)));
+    for (int t = 0; t < 7; ++t) {
         for (int rep = -1; rep < nrep; ++rep) {
             if (rep == 0) RS_HIP(hipEventRecord(a, c.stream));
             if (c.prec == 0) {
-                if (flags[t] >= 0) RS_TRY(hpsi_apply<double>(psiT<double>(), c.hp.as<double>(), c.MB, flags[t]));
-                else if (flags[t] == -1) RS_TRY(hpsi_stencil_only<double>(psiT<double>(), c.hp.as<double>(), c.MB));
-                else RS_TRY(nl_project<double>(psiT<double>(), c.MB, c.coefC.as<double>()));
+                double *x = psiT<double>(), *y = c.hp.as<double>(), *cf = c.coefC.as<double>();
+                if (flags[t] >= 0) RS_TRY(hpsi_apply<double>(x, y, c.MB, flags[t]));
+                else if (flags[t] == -1) RS_TRY(hpsi_stencil_only<double>(x, y, c.MB));
+                else if (flags[t] == -2) RS_TRY(nl_project<double>(x, c.MB, cf));
+                else RS_TRY(nl_apply<double>(y, c.MB, cf));
             } else {
-                if (flags[t] >= 0) RS_TRY(hpsi_apply<float>(psiT<float>(), c.hp.as<float>(), c.MB, flags[t]));
-                else if (flags[t] == -1) RS_TRY(hpsi_stencil_only<float>(psiT<float>(), c.hp.as<float>(), c.MB));
-                else RS_TRY(nl_project<float>(psiT<float>(), c.MB, c.coefC.as<float>()));
+                float *x = psiT<float>(), *y = c.hp.as<float>(), *cf = c.coefC.as<float>();
+                if (flags[t] >= 0) RS_TRY(hpsi_apply<float>(x, y, c.MB, flags[t]));
+                else if (flags[t] == -1) RS_TRY(hpsi_stencil_only<float>(x, y, c.MB));
+                else if (flags[t] == -2) RS_TRY(nl_project<float>(x, c.MB, cf));
+                else RS_TRY(nl_apply<float>(y, c.MB, cf));
             }
         }
         RS_HIP(hipEventRecord(b, c.stream));
@@ -683,7 +710,7 @@ int rsdft_gpu_finalize(void)
                      &c.mat3, &c.eval, &c.einfo, &c.evwork, &c.matT, &c.dots, &c.gemm_part, &c.opbuf,
                      &c.cg.buf, &c.cg.flags, &c.nl.row_ptr, &c.nl.row_idx, &c.nl.row_val64,
                      &c.nl.row_val32, &c.nl.col_ptr, &c.nl.col_lma, &c.nl.col_val64,
-                     &c.nl.col_val32, &c.nl.scale64, &c.nl.d_ptr, &c.nl.d_idx, &c.nl.d_val,
+                     &c.nl.col_val32, &c.nl.scale64, &c.nl.d_ptr, &c.nl.d_idx, &c.nl.d_val, &c.nl.cov_pts, &c.nl.cov_ptr,
                      &c.nl.amap_dev};
     for (DevMem *m : all) m->release();
     if (c.blas) rocblas_destroy_handle(c.blas);

@@ -462,7 +462,7 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
     const int nch = nchunk_of(ML);
     RS_TRY(c.cg.ensure(nb));
     // one size for every reduction of the sweep, so no later ensure() moves it
-    const int maxparts = std::max(nch, std::max(c.geom_pc.tiles, c.geom_h.tiles));
+    const int maxparts = std::max(nch, c.geom_pc.tiles);
     RS_TRY(c.part.ensure(sizeof(double) * (size_t)6 * nb * maxparts));
     double *part = c.part.as<double>();
     CGView v = view(c.cg);

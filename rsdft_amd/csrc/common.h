@@ -84,6 +84,9 @@ struct NonlocalTables {
     DevMem row_ptr, row_idx, row_val64, row_val32;
     DevMem col_ptr, col_lma, col_val64, col_val32;
     DevMem scale64;   // iuV(lma) * dV
+    int ncov = 0;             // grid points inside some sphere
+    int max_blk_entries = 0;  // max entries over blocks of 256 covered points
+    DevMem cov_pts, cov_ptr;  // covered points and their entry ranges in col_*
     // force tables
     int natom = 0;
     long nnzd = 0;
@@ -120,7 +123,8 @@ struct Context {
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
     GridDesc g;
-    StencilGeom geom_h;    // Md-order hamiltonian stencil
+    StencilGeom geom_h;    // Md-order hamiltonian stencil, FP64 (2 points per thread)
+    StencilGeom geom_h32;  // Md-order hamiltonian stencil, FP32 (4 points per thread)
     StencilGeom geom_pc;   // 7-point preconditioner stencil
     int prec = 0;
     int ncg = 3, mloop = 3;
@@ -133,7 +137,7 @@ struct Context {
     int MB = 0;
     int wf_prec = -1;
     DevMem psi, hx, hp, gk, pgk, pk, rk, ppc, qpc, tmp;
-    DevMem coefC;      // projection coefficients [band][lma]
+    DevMem coefC;      // projection coefficients [lma][band]
     DevMem part;       // reduction partial sums
     DevMem stage;      // host<->device staging (FP64)
     DevMem opbuf;      // working-precision staging of the op_* entry points
@@ -159,9 +163,12 @@ template <class T>
 int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active = nullptr);
 template <class T>
 int nl_project(const T *in, int nb, T *coef, const int *band_active = nullptr);
-// the fused stencil kernel alone, reusing the projection coefficients in place
+// the stencil kernel alone (kinetic + local, overwrite)
 template <class T>
 int hpsi_stencil_only(const T *in, T *out, int nb);
+// out += V_NL in from the projection coefficients coef[lma][band]
+template <class T>
+int nl_apply(T *out, int nb, const T *coef, const int *band_active = nullptr);
 
 // preconditioner (precond_hprsdft) for all active bands: in g, in/out pg.
 template <class T>

@@ -9,7 +9,8 @@
 // becomes H Psi (batched), one GEMM for Psi^T H Psi, the dsyr2k
 // symmetrisation, an FP64 eigensolve on the GPU and one rotation GEMM.
 // Plain GEMMs go to rocBLAS (atomics disabled, so results are
-// deterministic); the BF16 hi+lo split GEMMs are hand-written MFMA kernels.
+// deterministic); the FP32-storage GEMMs are hand-written MFMA kernels
+// (FP32 or BF16 hi+lo products, FP64-flushed accumulation).
 #include "common.h"
 
 namespace rsdft {
@@ -64,6 +65,16 @@ __global__ void k_zero_lower(double *A, int n)
     if (i < n && i > j) A[i + (long)j * n] = 0.0;
 }
 
+// eigenvector sign convention: V(j,j) >= 0 (any sign is an eigenvector; this
+// one keeps V close to I for a converging sweep)
+__global__ void k_fix_signs(double *V, int n)
+{
+    const int j = blockIdx.x;
+    const bool flip = V[(long)j * n + j] < 0.0;
+    if (!flip) return;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) V[(long)j * n + i] = -V[(long)j * n + i];
+}
+
 __global__ void k_check_info(const int *info, int *flag)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0 && *info != 0) *flag = *info;
@@ -82,14 +93,7 @@ static int gemm_tn(const T *A, const T *B, int nb, double alpha, double *C)
                               &z, C, nb));
         return kOK;
     }
-    if (c.prec == 2)
-        return split_gemm_tn<T>((const float *)A, (const float *)B, nb, ML, alpha, C);
-    RS_TRY(c.matT.ensure(sizeof(float) * (size_t)nb * nb));
-    const float a = (float)alpha, z = 0.0f;
-    RS_BLAS(rocblas_sgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb,
-                          (int)ML, &a, (const float *)A, (int)ML, (const float *)B, (int)ML, &z,
-                          c.matT.as<float>(), nb));
-    return convert_to_f64<float>(c.matT.as<float>(), C, (long)nb * nb);
+    return split_gemm_tn<T>((const float *)A, (const float *)B, nb, ML, alpha, C);
 }
 
 // Out(ML x nb) = A(ML x nb) * U(nb x nb), U FP64
@@ -105,14 +109,7 @@ static int gemm_nn(const T *A, const double *U, int nb, T *Out)
                               (int)ML));
         return kOK;
     }
-    if (c.prec == 2) return split_gemm_nn<T>((const float *)A, U, nb, ML, (float *)Out);
-    RS_TRY(c.matT.ensure(sizeof(float) * (size_t)nb * nb));
-    RS_TRY(convert_from_f64<float>(U, c.matT.as<float>(), (long)nb * nb));
-    const float a = 1.0f, z = 0.0f;
-    RS_BLAS(rocblas_sgemm(c.blas, rocblas_operation_none, rocblas_operation_none, (int)ML, nb, nb,
-                          &a, (const float *)A, (int)ML, c.matT.as<float>(), nb, &z, (float *)Out,
-                          (int)ML));
-    return kOK;
+    return split_gemm_nn<T>((const float *)A, U, nb, ML, (float *)Out);
 }
 
 static int ensure_dense(int nb)
@@ -165,6 +162,7 @@ int subspace_diag_dev(T *X, int nb, double *esp_dev)
     RS_BLAS(rocsolver_dsyevd(c.blas, rocblas_evect_original, rocblas_fill_upper, nb, H, nb, D,
                              D + nb, info));
     hipLaunchKernelGGL(k_check_info, dim3(1), dim3(1), 0, c.stream, info, info + 3);
+    hipLaunchKernelGGL(k_fix_signs, dim3(nb), dim3(128), 0, c.stream, H, nb);
     T *Out = c.tmp.as<T>();
     RS_TRY(gemm_nn<T>(X, H, nb, Out));
     RS_HIP(hipMemcpyAsync(esp_dev, D, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));

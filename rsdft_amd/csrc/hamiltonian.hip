@@ -12,10 +12,13 @@
 //   * periodic wrap in y/z is modular indexing of the global loads: the
 //     reference's bcset_3 halo exchange (bcset_3_module.f90:15-151,
 //     bc_module.f90:722-736) has no counterpart;
-//   * the epilogue adds Vloc*psi (op_localpot) and pulls the nonlocal term
-//     from a point-major CSR of the projector matrix, so H psi is written
-//     once.
+//   * the epilogue adds Vloc*psi (op_localpot);
+//   * the nonlocal term is added afterwards by k_nl_apply, which visits only
+//     the grid points inside some projector sphere, for many bands per
+//     entry load.
 #include "common.h"
+
+#include <algorithm>
 
 namespace rsdft {
 
@@ -40,17 +43,25 @@ __device__ __forceinline__ long xcd_work_index(long L, long total) {
 
 // ---------------------------------------------------------------------------
 // Md-order hamiltonian stencil with fused epilogue
+//
+// Thread t of a workgroup owns row r = t / tpr of the tile and the PX points
+// x = c + tpr*px (c = t % tpr), so consecutive lanes touch consecutive words
+// in LDS (no bank conflicts) and in HBM (coalesced).  The z queue holds
+// 2*MD+1+PF planes: plane z+MD+PF is requested PF steps before it is needed,
+// and the y-halo rows of plane z+1 are requested one step ahead into
+// registers, so no load latency sits between two barriers.
 // ---------------------------------------------------------------------------
-template <class T, int MD, int PX>
-__global__ void __launch_bounds__(1024)
+template <class T, int MD, int PX, int PF>
+__global__ void __launch_bounds__(512, 4)
 k_hpsi(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, int N3,
-       int ty, int tz, int tpr, StencilCoef<T, MD> cf, const T *__restrict__ vloc,
-       const int *__restrict__ cptr, const int *__restrict__ clma,
-       const T *__restrict__ cval, const T *__restrict__ coefC, int nz, int flags,
+       int ty, int tz, int tpr, StencilCoef<T, MD> cf, const T *__restrict__ vloc, int flags,
        const int *__restrict__ band_active)
 {
+    constexpr int QD = 2 * MD + 1 + PF;
+    constexpr int HMAX = 2;   // halo items per thread (ty >= MD guaranteed by the host)
     extern __shared__ unsigned char smem_raw[];
     T *L = reinterpret_cast<T *>(smem_raw);
+    const int W = N1 + 2 * MD;            // LDS row: [MD wrapped | N1 | MD wrapped]
     const int band = blockIdx.y;
     if (band_active && !band_active[band]) return;
     const int nty = N2 / ty;
@@ -59,89 +70,107 @@ k_hpsi(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, i
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     const int r = tid / tpr;
-    const int xs = (tid % tpr) * PX;
-    const bool own = r < ty;
-    const int yrow = y0 + (own ? r : 0);
+    const int cx0 = tid - r * tpr;
+    const int yrow = y0 + r;
     const long plane = (long)N1 * N2;
     const T *src = in + (long)band * ML;
     T *dst = out + (long)band * ML;
-    const bool kin = flags & kKin, loc = flags & kLoc, nl = flags & kNl, acc_out = flags & kAcc;
+    const bool kin = flags & kKin, loc = flags & kLoc, acc_out = flags & kAcc;
+    const long rowoff = (long)yrow * N1 + cx0;
 
-    T q[2 * MD + 1][PX];
-#pragma unroll
-    for (int k = 0; k < 2 * MD; ++k) {
-        const int zz = wrapi(z0 - MD + k, N3);
-        const T *p = src + zz * plane + (long)yrow * N1 + xs;
-#pragma unroll
-        for (int px = 0; px < PX; ++px) q[k][px] = p[px];
-    }
-    T *Lrow = L + (r + MD) * N1;
+    // halo items of this thread: (LDS row base, global row offset)
     const int nhalo = 2 * MD * tpr;
+    int h_lrow[HMAX], h_goff[HMAX], h_x[HMAX];
+    bool h_on[HMAX];
+#pragma unroll
+    for (int k = 0; k < HMAX; ++k) {
+        const int hr = tid + k * nthr;
+        h_on[k] = hr < nhalo;
+        const int hh = h_on[k] ? hr / tpr : 0;
+        const int c = hr - hh * tpr;
+        int yy = hh < MD ? y0 - MD + hh : y0 + ty + hh - MD;
+        yy = wrapi(yy, N2);
+        h_lrow[k] = (hh < MD ? hh : ty + hh) * W + MD + c;
+        h_goff[k] = yy * N1 + c;
+        h_x[k] = c;
+    }
+
+    T q[QD][PX];
+#pragma unroll
+    for (int k = 0; k < QD - 1; ++k) {
+        const int zz = wrapi(z0 - MD + k, N3);
+        const T *p = src + zz * plane + rowoff;
+#pragma unroll
+        for (int px = 0; px < PX; ++px) q[k][px] = p[px * tpr];
+    }
+    T hb[HMAX][PX];
+#pragma unroll
+    for (int k = 0; k < HMAX; ++k) {
+        if (!h_on[k]) continue;
+        const T *p = src + (long)z0 * plane + h_goff[k];
+#pragma unroll
+        for (int px = 0; px < PX; ++px) hb[k][px] = p[px * tpr];
+    }
+    T *Lrow = L + (r + MD) * W + MD + cx0;
     for (int z = z0; z < z0 + tz; ++z) {
-        {
-            int zn = z + MD;
+        {   // request plane z+MD+PF
+            int zn = z + MD + PF;
             if (zn >= N3) zn -= N3;
-            const T *p = src + zn * plane + (long)yrow * N1 + xs;
+            const T *p = src + zn * plane + rowoff;
 #pragma unroll
-            for (int px = 0; px < PX; ++px) q[2 * MD][px] = p[px];
+            for (int px = 0; px < PX; ++px) q[QD - 1][px] = p[px * tpr];
         }
-        if (own) {
+        // stage plane z: own row, the y halo rows held in registers, x pads
 #pragma unroll
-            for (int px = 0; px < PX; ++px) Lrow[xs + px] = q[MD][px];
+        for (int px = 0; px < PX; ++px) {
+            Lrow[px * tpr] = q[MD][px];
+            const int x = cx0 + px * tpr;
+            if (x < MD || x >= N1 - MD) Lrow[px * tpr + (x < MD ? N1 : -N1)] = q[MD][px];
         }
-        {
-            const T *zp = src + (long)z * plane;
-            for (int hr = tid; hr < nhalo; hr += nthr) {
-                const int hh = hr / tpr;
-                const int seg = (hr - hh * tpr) * PX;
-                int yy, lrow;
-                if (hh < MD) {
-                    yy = y0 - MD + hh;
-                    lrow = hh;
-                } else {
-                    yy = y0 + ty + hh - MD;
-                    lrow = ty + hh;
-                }
-                yy = wrapi(yy, N2);
-                const T *p = zp + (long)yy * N1 + seg;
-                T *d = L + lrow * N1 + seg;
 #pragma unroll
-                for (int px = 0; px < PX; ++px) d[px] = p[px];
-            }
-        }
-        __syncthreads();
-        if (own) {
+        for (int k = 0; k < HMAX; ++k) {
+            if (!h_on[k]) continue;
 #pragma unroll
             for (int px = 0; px < PX; ++px) {
-                const int x = xs + px;
-                T acc = T(0);
-                if (kin) {
-                    acc = cf.c0 * q[MD][px];
-#pragma unroll
-                    for (int m = 1; m <= MD; ++m) {
-                        int xm = x - m;
-                        if (xm < 0) xm += N1;
-                        int xp = x + m;
-                        if (xp >= N1) xp -= N1;
-                        acc = acc + cf.cx[m - 1] * (Lrow[xm] + Lrow[xp])
-                                  + cf.cy[m - 1] * (Lrow[xs + px - m * N1] + Lrow[xs + px + m * N1])
-                                  + cf.cz[m - 1] * (q[MD - m][px] + q[MD + m][px]);
-                    }
-                }
-                const long gi = (long)z * plane + (long)yrow * N1 + x;
-                if (loc) acc = acc + vloc[gi] * q[MD][px];
-                if (nl) {
-                    const int e1 = cptr[gi + 1];
-                    const T *cb = coefC + (long)band * nz;
-                    for (int e = cptr[gi]; e < e1; ++e) acc = acc + cval[e] * cb[clma[e]];
-                }
-                if (acc_out) acc = dst[gi] + acc;
-                dst[gi] = acc;
+                L[h_lrow[k] + px * tpr] = hb[k][px];
+                const int x = h_x[k] + px * tpr;
+                if (x < MD || x >= N1 - MD)
+                    L[h_lrow[k] + px * tpr + (x < MD ? N1 : -N1)] = hb[k][px];
             }
         }
         __syncthreads();
+        // request the halo rows of plane z+1
+        {
+            int zn = z + 1;
+            if (zn >= N3) zn -= N3;
 #pragma unroll
-        for (int k = 0; k < 2 * MD; ++k) {
+            for (int k = 0; k < HMAX; ++k) {
+                if (!h_on[k]) continue;
+                const T *p = src + zn * plane + h_goff[k];
+#pragma unroll
+                for (int px = 0; px < PX; ++px) hb[k][px] = p[px * tpr];
+            }
+        }
+#pragma unroll
+        for (int px = 0; px < PX; ++px) {
+            const T *lr = Lrow + px * tpr;
+            T acc = T(0);
+            if (kin) {
+                acc = cf.c0 * q[MD][px];
+#pragma unroll
+                for (int m = 1; m <= MD; ++m)
+                    acc = acc + cf.cx[m - 1] * (lr[-m] + lr[m])
+                              + cf.cy[m - 1] * (lr[-m * W] + lr[m * W])
+                              + cf.cz[m - 1] * (q[MD - m][px] + q[MD + m][px]);
+            }
+            const long gi = (long)z * plane + rowoff + px * tpr;
+            if (loc) acc = acc + vloc[gi] * q[MD][px];
+            if (acc_out) acc = dst[gi] + acc;
+            dst[gi] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < QD - 1; ++k) {
 #pragma unroll
             for (int px = 0; px < PX; ++px) q[k][px] = q[k + 1][px];
         }
@@ -149,7 +178,7 @@ k_hpsi(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, i
 }
 
 // ---------------------------------------------------------------------------
-// Nonlocal projection: coefC[b][lma] = iuV(lma)*dV*sum_j uV(j,lma) psi_b(JJP(j,lma))
+// Nonlocal projection: coefC[lma][b] = iuV(lma)*dV*sum_j uV(j,lma) psi_b(JJP(j,lma))
 // (ps_nloc2_op_module.f90:161-175), FP64 accumulation, NBB bands per block.
 // ---------------------------------------------------------------------------
 template <int NBB>
@@ -201,7 +230,63 @@ k_project(const T *__restrict__ in, long ML, int nb, int nz, const int *__restri
     if (threadIdx.x < NBB) {
         const int b = b0 + threadIdx.x;
         if (b < nb && (!band_active || band_active[b]))
-            coefC[(long)b * nz + lma] = (T)(scale[lma] * acc[0]);
+            coefC[(long)lma * nb + b] = (T)(scale[lma] * acc[0]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Nonlocal apply (ps_nloc2_op_module.f90:255-263):
+//   htpsi(i,b) += sum_{lma covering i} uV(i,lma) * coefC[lma][b]
+// over the grid points that lie in some sphere ("covered" points).  A
+// workgroup stages the point-major entries (lma, uV) of its 256 points in LDS
+// once and sweeps its band range NBB bands at a time, so the tables are read
+// once per workgroup instead of once per band.  Entries of a point are summed
+// in lma order, as the reference accumulates them.
+// ---------------------------------------------------------------------------
+template <class T, int NBB>
+__global__ void __launch_bounds__(256)
+k_nl_apply(T *__restrict__ out, long ML, int nb, int bands_per_group,
+           const int *__restrict__ cov_pts, const int *__restrict__ cov_ptr, int ncov,
+           const int *__restrict__ clma, const T *__restrict__ cval,
+           const T *__restrict__ coefC, const int *__restrict__ band_active)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int p0 = blockIdx.x * 256;
+    const int npts = min(256, ncov - p0);
+    const int e0 = cov_ptr[p0], e1 = cov_ptr[p0 + npts];
+    const int ne = e1 - e0;
+    int *Llma = reinterpret_cast<int *>(smem_raw);
+    T *Lval = reinterpret_cast<T *>(smem_raw + (((size_t)ne * sizeof(int) + 15) & ~(size_t)15));
+    for (int e = threadIdx.x; e < ne; e += 256) {
+        Llma[e] = clma[e0 + e];
+        Lval[e] = cval[e0 + e];
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t >= npts) return;
+    const long pt = cov_pts[p0 + t];
+    const int le0 = cov_ptr[p0 + t] - e0, le1 = cov_ptr[p0 + t + 1] - e0;
+    const int bg0 = blockIdx.y * bands_per_group;
+    const int bg1 = min(nb, bg0 + bands_per_group);
+    for (int b0 = bg0; b0 < bg1; b0 += NBB) {
+        T acc[NBB];
+#pragma unroll
+        for (int k = 0; k < NBB; ++k) acc[k] = T(0);
+        for (int e = le0; e < le1; ++e) {
+            const T v = Lval[e];
+            const T *cr = coefC + (long)Llma[e] * nb + b0;
+#pragma unroll
+            for (int k = 0; k < NBB; ++k)
+                if (b0 + k < bg1) acc[k] = acc[k] + v * cr[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NBB; ++k) {
+            const int b = b0 + k;
+            if (b < bg1 && (!band_active || band_active[b])) {
+                T *o = out + (long)b * ML + pt;
+                *o = *o + acc[k];
+            }
+        }
     }
 }
 
@@ -212,17 +297,18 @@ k_project(const T *__restrict__ in, long ML, int nb, int nz, const int *__restri
 //   c = sum_d 1/h_d^2, c_d = -0.5/h_d^2.
 // mode 0:  f = g:   Q = A g; R = g - Q; PP = R; PG = g; part = sum R^2
 // mode 1:  f = PP:  Q = A PP;                    part = sum PP*Q
+// Same lane-strided z-march as k_hpsi with a 1-plane prefetch.
 // ---------------------------------------------------------------------------
 template <class T, int PX>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(512)
 k_pc_stencil(const T *__restrict__ in, T *__restrict__ Q, T *__restrict__ R,
-             T *__restrict__ PP, T *__restrict__ PG, long ML, int N1, int N2, int N3, int ty, int tz, int tpr,
-             T c0, T c1, T c2, T c3, int mode, const int *__restrict__ band_active,
-             double *__restrict__ part, int tiles)
+             T *__restrict__ PP, T *__restrict__ PG, long ML, int N1, int N2, int N3, int ty,
+             int tz, int tpr, T c0, T c1, T c2, T c3, int mode,
+             const int *__restrict__ band_active, double *__restrict__ part, int tiles)
 {
     extern __shared__ unsigned char smem_raw[];
     T *L = reinterpret_cast<T *>(smem_raw);
-    __shared__ double red[16];
+    __shared__ double red[8];
     const int band = blockIdx.y;
     if (band_active && !band_active[band]) return;
     const int nty = N2 / ty;
@@ -230,71 +316,88 @@ k_pc_stencil(const T *__restrict__ in, T *__restrict__ Q, T *__restrict__ R,
     const int z0 = (blockIdx.x / nty) * tz;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int r = tid / tpr;
-    const int xs = (tid % tpr) * PX;
-    const bool own = r < ty;
-    const int yrow = y0 + (own ? r : 0);
+    const int cx0 = tid - r * tpr;
+    const int yrow = y0 + r;
     const long plane = (long)N1 * N2;
     const long boff = (long)band * ML;
     const T *src = in + boff;
-    T qm[PX], q0[PX], qp[PX];
+    const long rowoff = (long)yrow * N1 + cx0;
+    // halo: rows y0-1 and y0+ty, 2*tpr items, at most one per thread (ty >= 2 or tpr*ty >= 2*tpr)
+    const int nhalo = 2 * tpr;
+    const bool h_on = tid < nhalo;
+    const int hh = h_on ? tid / tpr : 0;
+    const int hc = tid - hh * tpr;
+    const int h_lrow = (hh == 0 ? 0 : ty + 1) * N1 + hc;
+    const int h_goff = wrapi(hh == 0 ? y0 - 1 : y0 + ty, N2) * N1 + hc;
+
+    T qm[PX], q0[PX], qp[PX], qn[PX], hb[PX];
     {
-        const int zz = wrapi(z0 - 1, N3);
-        const T *p = src + zz * plane + (long)yrow * N1 + xs;
-        const T *p0 = src + (long)z0 * plane + (long)yrow * N1 + xs;
+        const T *pm = src + wrapi(z0 - 1, N3) * plane + rowoff;
+        const T *p0 = src + (long)z0 * plane + rowoff;
+        const T *pp = src + wrapi(z0 + 1, N3) * plane + rowoff;
 #pragma unroll
-        for (int px = 0; px < PX; ++px) { qm[px] = p[px]; q0[px] = p0[px]; }
+        for (int px = 0; px < PX; ++px) {
+            qm[px] = pm[px * tpr];
+            q0[px] = p0[px * tpr];
+            qp[px] = pp[px * tpr];
+        }
+        if (h_on) {
+            const T *ph = src + (long)z0 * plane + h_goff;
+#pragma unroll
+            for (int px = 0; px < PX; ++px) hb[px] = ph[px * tpr];
+        }
     }
-    T *Lrow = L + (r + 1) * N1;
+    T *Lrow = L + (r + 1) * N1 + cx0;
     double sum = 0.0;
     for (int z = z0; z < z0 + tz; ++z) {
         {
-            int zn = z + 1;
+            int zn = z + 2;
             if (zn >= N3) zn -= N3;
-            const T *p = src + zn * plane + (long)yrow * N1 + xs;
+            const T *p = src + zn * plane + rowoff;
 #pragma unroll
-            for (int px = 0; px < PX; ++px) qp[px] = p[px];
+            for (int px = 0; px < PX; ++px) qn[px] = p[px * tpr];
         }
-        if (own) {
 #pragma unroll
-            for (int px = 0; px < PX; ++px) Lrow[xs + px] = q0[px];
-        }
-        for (int hr = tid; hr < 2 * tpr; hr += nthr) {
-            const int hh = hr / tpr;
-            const int seg = (hr - hh * tpr) * PX;
-            const int yy = wrapi(hh == 0 ? y0 - 1 : y0 + ty, N2);
-            const int lrow = hh == 0 ? 0 : ty + 1;
-            const T *p = src + (long)z * plane + (long)yy * N1 + seg;
+        for (int px = 0; px < PX; ++px) Lrow[px * tpr] = q0[px];
+        if (h_on) {
 #pragma unroll
-            for (int px = 0; px < PX; ++px) L[lrow * N1 + seg + px] = p[px];
+            for (int px = 0; px < PX; ++px) L[h_lrow + px * tpr] = hb[px];
         }
         __syncthreads();
-        if (own) {
+        if (h_on) {
+            int zn = z + 1;
+            if (zn >= N3) zn -= N3;
+            const T *p = src + zn * plane + h_goff;
 #pragma unroll
-            for (int px = 0; px < PX; ++px) {
-                const int x = xs + px;
-                int xm = x - 1;
-                if (xm < 0) xm += N1;
-                int xp = x + 1;
-                if (xp >= N1) xp -= N1;
-                T f = c0 * q0[px];
-                f = f + c1 * (Lrow[xp] + Lrow[xm]) + c2 * (Lrow[x + N1] + Lrow[x - N1])
-                      + c3 * (qp[px] + qm[px]);
-                const long gi = boff + (long)z * plane + (long)yrow * N1 + x;
-                if (mode == 0) {
-                    const T rr = q0[px] - f;
-                    R[gi] = rr;
-                    PP[gi] = rr;
-                    PG[gi] = q0[px];
-                    sum += (double)rr * (double)rr;
-                } else {
-                    Q[gi] = f;
-                    sum += (double)q0[px] * (double)f;
-                }
+            for (int px = 0; px < PX; ++px) hb[px] = p[px * tpr];
+        }
+#pragma unroll
+        for (int px = 0; px < PX; ++px) {
+            const int x = cx0 + px * tpr;
+            const int dm = x == 0 ? N1 - 1 : -1;
+            const int dp = x == N1 - 1 ? 1 - N1 : 1;
+            const T *lr = Lrow + px * tpr;
+            T f = c0 * q0[px];
+            f = f + c1 * (lr[dp] + lr[dm]) + c2 * (lr[N1] + lr[-N1]) + c3 * (qp[px] + qm[px]);
+            const long gi = boff + (long)z * plane + rowoff + px * tpr;
+            if (mode == 0) {
+                const T rr = q0[px] - f;
+                R[gi] = rr;
+                PP[gi] = rr;
+                PG[gi] = q0[px];
+                sum += (double)rr * (double)rr;
+            } else {
+                Q[gi] = f;
+                sum += (double)q0[px] * (double)f;
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int px = 0; px < PX; ++px) { qm[px] = q0[px]; q0[px] = qp[px]; }
+        for (int px = 0; px < PX; ++px) {
+            qm[px] = q0[px];
+            q0[px] = qp[px];
+            qp[px] = qn[px];
+        }
     }
     const int lane = tid & 63, wave = tid >> 6, nw = (nthr + 63) >> 6;
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off, 64);
@@ -316,11 +419,12 @@ StencilGeom choose_geom(int N1, int N2, int N3, int md, int px, int elem_bytes)
     g.px = px;
     const int tpr = N1 / px;
     double best = -1.0;
-    for (int ty = 1; ty <= N2; ++ty) {
+    g.threads = 0;
+    for (int ty = md; ty <= N2; ++ty) {   // ty >= md keeps halo items <= 2 per thread
         if (N2 % ty) continue;
         const int thr = tpr * ty;
         if (thr > 512) break;
-        const size_t lds = (size_t)(ty + 2 * md) * N1 * elem_bytes;
+        const size_t lds = (size_t)(ty + 2 * md) * (N1 + 2 * md) * elem_bytes;
         if (lds > 48 * 1024) continue;
         const int waves = (thr + 63) / 64;
         const double eff = (double)thr / (64.0 * waves);
@@ -350,11 +454,12 @@ StencilGeom choose_geom(int N1, int N2, int N3, int md, int px, int elem_bytes)
 
 template <class T, int MD>
 static int launch_hpsi_md(const T *in, T *out, int nb, int flags, const int *band_active,
-                          const T *vloc, const T *cval, const T *coefC)
+                          const T *vloc)
 {
     Context &c = ctx();
     const GridDesc &g = c.g;
-    const StencilGeom &G = c.geom_h;
+    const StencilGeom &G = sizeof(T) == 8 ? c.geom_h : c.geom_h32;
+    constexpr int PX = sizeof(T) == 8 ? 2 : 4;
     StencilCoef<T, MD> cf;
     cf.c0 = (T)g.coef_lap0;
     for (int m = 0; m < MD; ++m) {
@@ -362,14 +467,30 @@ static int launch_hpsi_md(const T *in, T *out, int nb, int flags, const int *ban
         cf.cy[m] = (T)g.coef_lap[1][m];
         cf.cz[m] = (T)g.coef_lap[2][m];
     }
-    const int tpr = g.N1 / G.px;
-    const size_t lds = (size_t)(G.ty + 2 * MD) * g.N1 * sizeof(T);
+    const int tpr = g.N1 / PX;
+    const size_t lds = (size_t)(G.ty + 2 * MD) * (g.N1 + 2 * MD) * sizeof(T);
     dim3 grid(G.tiles, nb);
-    hipLaunchKernelGGL((k_hpsi<T, MD, 4>), grid, dim3(G.threads), lds, c.stream, in, out, g.ML,
-                       g.N1, g.N2, g.N3, G.ty, G.tz, tpr, cf, vloc, c.nl.col_ptr.as<int>(),
-                       c.nl.col_lma.as<int>(), cval, coefC, c.nl.nz, flags, band_active);
+    hipLaunchKernelGGL((k_hpsi<T, MD, PX, 2>), grid, dim3(G.threads), lds, c.stream, in, out, g.ML,
+                       g.N1, g.N2, g.N3, G.ty, G.tz, tpr, cf, vloc, flags, band_active);
     RS_HIP(hipGetLastError());
     return kOK;
+}
+
+template <class T>
+static int launch_stencil(const T *in, T *out, int nb, int flags, const int *band_active)
+{
+    Context &c = ctx();
+    const T *vloc = sizeof(T) == 8 ? (const T *)c.vloc64.p : (const T *)c.vloc32.p;
+    if ((flags & kLoc) && !vloc) return set_error(kErrState, "local potential not set");
+    switch (c.g.Md) {
+    case 1: return launch_hpsi_md<T, 1>(in, out, nb, flags, band_active, vloc);
+    case 2: return launch_hpsi_md<T, 2>(in, out, nb, flags, band_active, vloc);
+    case 3: return launch_hpsi_md<T, 3>(in, out, nb, flags, band_active, vloc);
+    case 4: return launch_hpsi_md<T, 4>(in, out, nb, flags, band_active, vloc);
+    case 6: return launch_hpsi_md<T, 6>(in, out, nb, flags, band_active, vloc);
+    case 8: return launch_hpsi_md<T, 8>(in, out, nb, flags, band_active, vloc);
+    default: return set_error(kErrArg, "Md=%d not compiled (1,2,3,4,6,8)", c.g.Md);
+    }
 }
 
 template <class T>
@@ -388,42 +509,53 @@ int nl_project(const T *in, int nb, T *coef, const int *band_active)
 }
 
 template <class T>
+int nl_apply(T *out, int nb, const T *coef, const int *band_active)
+{
+    Context &c = ctx();
+    NonlocalTables &nl = c.nl;
+    if (nl.nz <= 0 || nl.ncov <= 0) return kOK;
+    constexpr int NBB = sizeof(T) == 8 ? 8 : 16;
+    const int nblk = (nl.ncov + 255) / 256;
+    // enough workgroups for 256 CUs: split the bands into groups
+    int ngroups = std::max(1, std::min((nb + NBB - 1) / NBB, (2048 + nblk - 1) / nblk));
+    int bpg = (nb + ngroups - 1) / ngroups;
+    bpg = (bpg + NBB - 1) / NBB * NBB;
+    ngroups = (nb + bpg - 1) / bpg;
+    const size_t lds = (((size_t)nl.max_blk_entries * sizeof(int) + 15) & ~(size_t)15) +
+                       (size_t)nl.max_blk_entries * sizeof(T);
+    const T *cval = sizeof(T) == 8 ? (const T *)nl.col_val64.p : (const T *)nl.col_val32.p;
+    hipLaunchKernelGGL((k_nl_apply<T, NBB>), dim3(nblk, ngroups), dim3(256), lds, c.stream, out,
+                       c.g.ML, nb, bpg, nl.cov_pts.as<int>(), nl.cov_ptr.as<int>(), nl.ncov,
+                       nl.col_lma.as<int>(), cval, coef, band_active);
+    RS_HIP(hipGetLastError());
+    return kOK;
+}
+
+template <class T>
 int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
 {
     Context &c = ctx();
     if (nb <= 0) return kOK;
-    const bool have_nl = c.nl.nz > 0;
-    if (!have_nl) flags &= ~kNl;
-    const T *coefC = nullptr;
+    if (c.nl.nz <= 0) flags &= ~kNl;
+    const int sflags = flags & (kKin | kLoc | kAcc);
     if (flags & kNl) {
         RS_TRY(c.coefC.ensure((size_t)nb * c.nl.nz * sizeof(T)));
         RS_TRY(nl_project<T>(in, nb, c.coefC.as<T>(), band_active));
-        coefC = c.coefC.as<T>();
     }
-    const T *vloc = sizeof(T) == 8 ? (const T *)c.vloc64.p : (const T *)c.vloc32.p;
-    const T *cval = sizeof(T) == 8 ? (const T *)c.nl.col_val64.p : (const T *)c.nl.col_val32.p;
-    if ((flags & kLoc) && !vloc) return set_error(kErrState, "local potential not set");
+    if (sflags & (kKin | kLoc)) {
+        RS_TRY(launch_stencil<T>(in, out, nb, sflags, band_active));
+    } else if (!(flags & kAcc)) {
+        RS_HIP(hipMemsetAsync(out, 0, sizeof(T) * (size_t)c.g.ML * nb, c.stream));
+    }
+    if (flags & kNl) RS_TRY(nl_apply<T>(out, nb, c.coefC.as<T>(), band_active));
     c.hpsi_bands += nb;
-    switch (c.g.Md) {
-    case 1: return launch_hpsi_md<T, 1>(in, out, nb, flags, band_active, vloc, cval, coefC);
-    case 2: return launch_hpsi_md<T, 2>(in, out, nb, flags, band_active, vloc, cval, coefC);
-    case 3: return launch_hpsi_md<T, 3>(in, out, nb, flags, band_active, vloc, cval, coefC);
-    case 4: return launch_hpsi_md<T, 4>(in, out, nb, flags, band_active, vloc, cval, coefC);
-    case 6: return launch_hpsi_md<T, 6>(in, out, nb, flags, band_active, vloc, cval, coefC);
-    case 8: return launch_hpsi_md<T, 8>(in, out, nb, flags, band_active, vloc, cval, coefC);
-    default: return set_error(kErrArg, "Md=%d not compiled (1,2,3,4,6,8)", c.g.Md);
-    }
+    return kOK;
 }
 
 template <class T>
 int hpsi_stencil_only(const T *in, T *out, int nb)
 {
-    Context &c = ctx();
-    const T *vloc = sizeof(T) == 8 ? (const T *)c.vloc64.p : (const T *)c.vloc32.p;
-    const T *cval = sizeof(T) == 8 ? (const T *)c.nl.col_val64.p : (const T *)c.nl.col_val32.p;
-    const int flags = kKin | kLoc | (c.nl.nz > 0 ? kNl : 0);
-    if (c.g.Md != 6) return set_error(kErrArg, "stencil-only timing is built for Md=6");
-    return launch_hpsi_md<T, 6>(in, out, nb, flags, nullptr, vloc, cval, c.coefC.as<T>());
+    return launch_stencil<T>(in, out, nb, kKin | kLoc, nullptr);
 }
 
 template <class T>
@@ -450,6 +582,8 @@ template int hpsi_apply<float>(const float *, float *, int, int, const int *);
 template int hpsi_stencil_only<double>(const double *, double *, int);
 template int hpsi_stencil_only<float>(const float *, float *, int);
 template int nl_project<double>(const double *, int, double *, const int *);
+template int nl_apply<double>(double *, int, const double *, const int *);
+template int nl_apply<float>(float *, int, const float *, const int *);
 template int nl_project<float>(const float *, int, float *, const int *);
 template int pc_stencil<double>(const double *, double *, double *, double *, double *, int,
                                 int, const int *, double *);

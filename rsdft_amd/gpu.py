@@ -254,8 +254,8 @@ class RsdftGpu:
     # -- measurement ---------------------------------------------------------------------
     def test_hpsi(self, nrep: int = 10):
         """ms per all-band application: hamiltonian, kinetic, localpot, nonlocal,
-        fused stencil kernel alone, projection kernel alone."""
-        t = np.zeros(6)
+        stencil kernel, projection kernel, nonlocal apply kernel."""
+        t = np.zeros(7)
         self._call("rsdft_gpu_test_hpsi", nrep, _p(t))
         return t
 
