@@ -382,3 +382,26 @@ def make_system(ncell: int, nband: int, seed: int = 20240601, Md: int = 6,
 def make_config(name: str, seed: int = 20240601, nband: int | None = None) -> System:
     ncell, nb, disp = CONFIGS[name]
     return make_system(ncell, nb if nband is None else nband, seed=seed, displace=disp, name=name)
+
+
+def make_custom(N, L, positions, Md: int = 6, nband: int = 8, seed: int = 7,
+                nocc: int | None = None, name: str = "custom", D=None) -> System:
+    """System on an orthorhombic N[0]xN[1]xN[2] grid of a cell with edges L,
+    atoms at Cartesian ``positions`` (bohr, shape (natom, 3))."""
+    grid = Grid(tuple(int(n) for n in N), tuple(float(x) for x in L))
+    h = np.array(grid.h)
+    pos = np.atleast_2d(np.asarray(positions, dtype=np.float64))
+    ic = np.rint(pos / h[None, :]).astype(np.int64)
+    delta = pos - ic * h[None, :]
+    ic = np.mod(ic, np.array(grid.N)[None, :])
+    natom = len(ic)
+    if D is None:
+        D = 0.5 + 1.5 * splitmix_uniform(seed, STREAM_D, natom * 4).reshape(natom, 4)
+    vloc, _, _ = make_vloc(grid, seed)
+    proj = make_projectors(grid, ic, delta, D)
+    if nocc is None:
+        nocc = min(2 * natom, nband)
+    occ = np.zeros(nband)
+    occ[:nocc] = 2.0
+    c0, cl = kinetic_coefficients(Md, grid.h)
+    return System(name, grid, Md, ic, delta, np.asarray(D), vloc, proj, nband, occ, seed, c0, cl)
