@@ -38,6 +38,32 @@ def _compile(src: str) -> str:
     return obj
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FDIR = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(BUILD, "fortran")
+DRIVER = os.path.join(FBUILD, "sweep_driver")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Compile the ISO_C_BINDING module and the sweep driver against the library."""
+    os.makedirs(FBUILD, exist_ok=True)
+    mod_obj = os.path.join(FBUILD, "rsdft_gpu_mod.o")
+    steps = [
+        [FLANG, "-O2", "-fopenmp", "-c", os.path.join(FDIR, "rsdft_gpu_mod.f90"), "-J", FBUILD,
+         "-o", mod_obj],
+        [FLANG, "-O2", "-fopenmp", os.path.join(FDIR, "sweep_driver.f90"), "-J", FBUILD,
+         "-I", FBUILD, mod_obj, "-o", DRIVER, "-L" + HERE, "-lrsdft_gpu",
+         "-Wl,-rpath," + HERE, "-Wl,-rpath,/opt/rocm/lib"],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {DRIVER}")
+    return DRIVER
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
@@ -52,6 +78,8 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    if os.path.exists(FLANG):
+        build_fortran(verbose)
     return LIB
 
 
