@@ -143,9 +143,13 @@ def main():
     _, espm, _ = dev.sweep(xm, 2)
     dev.detach()
     occ = sysm.occ
-    eig_err = float(np.max(np.abs(espm - esp64)))
+    err = np.abs(espm - esp64)
+    eig_err = float(np.max(err))
+    eig_err_occ = float(np.max(err[occ > 0]))
+    worst = np.argsort(err)[::-1][:5]
     eband_err = float(abs(np.sum(occ * espm) - np.sum(occ * esp64)))
-    log(f"eigenvalue error after 2 FP64 refinement sweeps: {eig_err:.3e} Ha, band energy {eband_err:.3e}")
+    log(f"eigenvalue error after 2 FP64 refinement sweeps: {eig_err:.3e} Ha (occupied {eig_err_occ:.3e}), "
+        f"band energy {eband_err:.3e}; worst bands {list(worst)} errs {[f'{err[i]:.2e}' for i in worst]}")
 
     # ---- roofline of the dominant kernel (fused hpsi stencil, FP32) ---------
     t_kernel_ms = float(hpm[4])
@@ -179,6 +183,7 @@ def main():
             "fp64_s_per_sweep": t_fp64,
             "fp64_over_mixed": t_fp64 / t_mix,
             "max_eig_err_ha": eig_err,
+            "max_eig_err_occupied_ha": eig_err_occ,
             "band_energy_err_ha": eband_err,
             "hpsi_ms_fp32": float(hpm[0]),
             "hpsi_ms_fp64": float(hp64[0]),

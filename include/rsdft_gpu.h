@@ -64,6 +64,12 @@ RSDFT_API int rsdft_gpu_set_precision(int mode);
 /* Ncg (cg_module.f90:23, keyword NCG) and mloop (cgpc_module.f90:95-96). */
 RSDFT_API int rsdft_gpu_set_solver_params(int ncg, int mloop);
 
+/* How precond_hprsdft (cgpc_hprsdft_module.f90:126-263) is evaluated:
+ * 1 (default) s-step form -- the same mloop CG steps replayed in the
+ * coefficient space of a Chebyshev Krylov basis from one fused Gram pass
+ * (mloop <= 4); 0 -- the step-by-step recurrence. */
+RSDFT_API int rsdft_gpu_set_precond_mode(int mode);
+
 /* hamiltonian(k,s,tpsi,htpsi,n1,n2,ib1,ib2) (hamiltonian_module.f90:20-72):
  * htpsi = T tpsi + Vloc tpsi + V_NL tpsi (htpsi overwritten).  Gamma only:
  * k must be 1, s must be 1. */

@@ -394,6 +394,13 @@ int rsdft_gpu_set_precision(int mode)
     return kOK;
 }
 
+int rsdft_gpu_set_precond_mode(int mode)
+{
+    if (mode < 0 || mode > 1) return set_error(kErrArg, "preconditioner mode %d", mode);
+    ctx().pc_mode = mode;
+    return kOK;
+}
+
 int rsdft_gpu_set_solver_params(int ncg, int mloop)
 {
     if (ncg < 0 || mloop < 0) return set_error(kErrArg, "ncg=%d mloop=%d", ncg, mloop);
@@ -707,7 +714,7 @@ int rsdft_gpu_finalize(void)
     (void)hipStreamSynchronize(c.stream);
     DevMem *all[] = {&c.vloc64, &c.vloc32, &c.psi, &c.hx, &c.hp, &c.gk, &c.pgk, &c.pk, &c.rk,
                      &c.ppc, &c.qpc, &c.tmp, &c.coefC, &c.part, &c.stage, &c.mat, &c.mat2,
-                     &c.mat3, &c.eval, &c.einfo, &c.evwork, &c.matT, &c.dots, &c.gemm_part, &c.opbuf,
+                     &c.mat3, &c.eval, &c.einfo, &c.evwork, &c.matT, &c.dots, &c.gemm_part, &c.opbuf, &c.pc_coef,
                      &c.cg.buf, &c.cg.flags, &c.nl.row_ptr, &c.nl.row_idx, &c.nl.row_val64,
                      &c.nl.row_val32, &c.nl.col_ptr, &c.nl.col_lma, &c.nl.col_val64,
                      &c.nl.col_val32, &c.nl.scale64, &c.nl.d_ptr, &c.nl.d_idx, &c.nl.d_val, &c.nl.cov_pts, &c.nl.cov_ptr,
@@ -724,6 +731,7 @@ int rsdft_gpu_finalize(void)
     c.prec = 0;
     c.nl.nz = 0;
     c.nl.natom = 0;
+    c.pc_mode = 1;
     for (double &x : c.timers.ms) x = 0.0;
     c.hpsi_bands = 0;
     return kOK;

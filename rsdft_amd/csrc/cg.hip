@@ -433,6 +433,12 @@ int precond_batched(const T *g, T *pg, int nb, const int *active)
         RS_HIP(hipGetLastError());
         return kOK;
     }
+    if (c.pc_mode == 1 && c.mloop <= 4) {
+        RS_TRY(c.part.ensure(sizeof(double) * std::max(precond_sstep_part_elems(nb),
+                                                        (size_t)6 * nb * nch)));
+        RS_TRY(c.pc_coef.ensure(sizeof(double) * (size_t)nb * c.mloop));
+        return precond_sstep<T>(g, pg, nb, active, c.part.as<double>(), c.pc_coef.as<double>());
+    }
     T *R = c.rk.as<T>(), *PP = c.ppc.as<T>(), *Q = c.qpc.as<T>();
     const int tiles = c.geom_pc.tiles;
     RS_TRY(c.part.ensure(sizeof(double) * (size_t)6 * nb * (nch > tiles ? nch : tiles)));
@@ -463,7 +469,9 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
     RS_TRY(c.cg.ensure(nb));
     // one size for every reduction of the sweep, so no later ensure() moves it
     const int maxparts = std::max(nch, c.geom_pc.tiles);
-    RS_TRY(c.part.ensure(sizeof(double) * (size_t)6 * nb * maxparts));
+    RS_TRY(c.part.ensure(std::max(sizeof(double) * (size_t)6 * nb * maxparts,
+                                  sizeof(double) * precond_sstep_part_elems(nb))));
+    RS_TRY(c.pc_coef.ensure(sizeof(double) * (size_t)nb * std::max(c.mloop, 1)));
     double *part = c.part.as<double>();
     CGView v = view(c.cg);
     T *HX = c.hx.as<T>(), *HP = c.hp.as<T>(), *G = c.gk.as<T>(), *PG = c.pgk.as<T>(),

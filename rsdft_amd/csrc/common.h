@@ -128,6 +128,7 @@ struct Context {
     StencilGeom geom_pc;   // 7-point preconditioner stencil
     int prec = 0;
     int ncg = 3, mloop = 3;
+    int pc_mode = 1;   // 0: step-by-step preconditioner recurrence, 1: s-step form
 
     DevMem vloc64, vloc32;
     NonlocalTables nl;
@@ -144,6 +145,7 @@ struct Context {
     DevMem mat, mat2, mat3, eval, einfo, evwork;   // nb x nb FP64 dense work
     DevMem matT;       // nb x nb in working precision
     DevMem dots;       // generic per-band result buffer (FP64)
+    DevMem pc_coef;    // s-step preconditioner coefficients [band][mloop]
     CGScalars cg;
     Timers timers;
     long hpsi_bands = 0;
@@ -173,6 +175,11 @@ int nl_apply(T *out, int nb, const T *coef, const int *band_active = nullptr);
 // preconditioner (precond_hprsdft) for all active bands: in g, in/out pg.
 template <class T>
 int precond_batched(const T *g, T *pg, int nb, const int *active);
+
+// s-step form of the same preconditioner (precond_sstep.hip)
+template <class T>
+int precond_sstep(const T *g, T *pg, int nb, const int *active, double *part, double *dcoef);
+size_t precond_sstep_part_elems(int nb);
 
 // reductions: out[k*nb + b] = scale * sum_i a_k[b,i]*b_k[b,i], k < ndot
 template <class T>

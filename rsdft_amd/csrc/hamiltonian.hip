@@ -54,19 +54,25 @@ __device__ __forceinline__ long xcd_work_index(long L, long total) {
 template <class T, int MD, int PX, int PF>
 __global__ void __launch_bounds__(512, 4)
 k_hpsi(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, int N3,
-       int ty, int tz, int tpr, StencilCoef<T, MD> cf, const T *__restrict__ vloc, int flags,
-       const int *__restrict__ band_active)
+       int ty, int tz, int tpr, int tiles, int nbands, StencilCoef<T, MD> cf,
+       const T *__restrict__ vloc, int flags, const int *__restrict__ band_active)
 {
     constexpr int QD = 2 * MD + 1 + PF;
     constexpr int HMAX = 2;   // halo items per thread (ty >= MD guaranteed by the host)
     extern __shared__ unsigned char smem_raw[];
     T *L = reinterpret_cast<T *>(smem_raw);
     const int W = N1 + 2 * MD;            // LDS row: [MD wrapped | N1 | MD wrapped]
-    const int band = blockIdx.y;
+    // tiles of one band go to one XCD back to back, so their shared y/z halo
+    // planes are served by that XCD's L2
+    const long total = (long)tiles * nbands;
+    const long wk = xcd_work_index(blockIdx.x, total);
+    if (wk >= total) return;
+    const int band = (int)(wk / tiles);
+    const int tile = (int)(wk - (long)band * tiles);
     if (band_active && !band_active[band]) return;
     const int nty = N2 / ty;
-    const int y0 = (blockIdx.x % nty) * ty;
-    const int z0 = (blockIdx.x / nty) * tz;
+    const int y0 = (tile % nty) * ty;
+    const int z0 = (tile / nty) * tz;
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     const int r = tid / tpr;
@@ -304,16 +310,20 @@ __global__ void __launch_bounds__(512)
 k_pc_stencil(const T *__restrict__ in, T *__restrict__ Q, T *__restrict__ R,
              T *__restrict__ PP, T *__restrict__ PG, long ML, int N1, int N2, int N3, int ty,
              int tz, int tpr, T c0, T c1, T c2, T c3, int mode,
-             const int *__restrict__ band_active, double *__restrict__ part, int tiles)
+             const int *__restrict__ band_active, double *__restrict__ part, int tiles, int nbands)
 {
     extern __shared__ unsigned char smem_raw[];
     T *L = reinterpret_cast<T *>(smem_raw);
     __shared__ double red[8];
-    const int band = blockIdx.y;
+    const long total = (long)tiles * nbands;
+    const long wk = xcd_work_index(blockIdx.x, total);
+    if (wk >= total) return;
+    const int band = (int)(wk / tiles);
+    const int tile = (int)(wk - (long)band * tiles);
     if (band_active && !band_active[band]) return;
     const int nty = N2 / ty;
-    const int y0 = (blockIdx.x % nty) * ty;
-    const int z0 = (blockIdx.x / nty) * tz;
+    const int y0 = (tile % nty) * ty;
+    const int z0 = (tile / nty) * tz;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int r = tid / tpr;
     const int cx0 = tid - r * tpr;
@@ -406,7 +416,7 @@ k_pc_stencil(const T *__restrict__ in, T *__restrict__ Q, T *__restrict__ R,
     if (tid == 0) {
         double s = 0.0;
         for (int w = 0; w < nw; ++w) s += red[w];
-        part[(long)band * tiles + blockIdx.x] = s;
+        part[(long)band * tiles + tile] = s;
     }
 }
 
@@ -469,9 +479,10 @@ static int launch_hpsi_md(const T *in, T *out, int nb, int flags, const int *ban
     }
     const int tpr = g.N1 / PX;
     const size_t lds = (size_t)(G.ty + 2 * MD) * (g.N1 + 2 * MD) * sizeof(T);
-    dim3 grid(G.tiles, nb);
-    hipLaunchKernelGGL((k_hpsi<T, MD, PX, 2>), grid, dim3(G.threads), lds, c.stream, in, out, g.ML,
-                       g.N1, g.N2, g.N3, G.ty, G.tz, tpr, cf, vloc, flags, band_active);
+    const long nblk = (long)G.tiles * nb;
+    hipLaunchKernelGGL((k_hpsi<T, MD, PX, 2>), dim3((nblk + 7) / 8 * 8), dim3(G.threads), lds,
+                       c.stream, in, out, g.ML, g.N1, g.N2, g.N3, G.ty, G.tz, tpr, G.tiles, nb, cf,
+                       vloc, flags, band_active);
     RS_HIP(hipGetLastError());
     return kOK;
 }
@@ -570,9 +581,10 @@ int pc_stencil(const T *in, T *Q, T *R, T *PP, T *PG, int nb, int mode, const in
     const T c0 = (T)(hx + hy + hz), c1 = (T)(-0.5 * hx), c2 = (T)(-0.5 * hy), c3 = (T)(-0.5 * hz);
     const int tpr = g.N1 / G.px;
     const size_t lds = (size_t)(G.ty + 2) * g.N1 * sizeof(T);
-    hipLaunchKernelGGL((k_pc_stencil<T, 4>), dim3(G.tiles, nb), dim3(G.threads), lds, c.stream,
-                       in, Q, R, PP, PG, g.ML, g.N1, g.N2, g.N3, G.ty, G.tz, tpr, c0, c1, c2, c3, mode,
-                       active, part, G.tiles);
+    const long nblk = (long)G.tiles * nb;
+    hipLaunchKernelGGL((k_pc_stencil<T, 4>), dim3((nblk + 7) / 8 * 8), dim3(G.threads), lds,
+                       c.stream, in, Q, R, PP, PG, g.ML, g.N1, g.N2, g.N3, G.ty, G.tz, tpr, c0, c1,
+                       c2, c3, mode, active, part, G.tiles, nb);
     RS_HIP(hipGetLastError());
     return kOK;
 }

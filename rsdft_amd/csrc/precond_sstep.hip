@@ -1,0 +1,363 @@
+// s-step form of the preconditioner precond_hprsdft
+// (cgpc_hprsdft_module.f90:126-263).
+//
+// The reference runs mloop CG steps on A x = g from x0 = g, where A is the
+// 7-point -1/2 Laplacian of precond_cg_mat2_omp (E = 0):
+//     r0 = g - A g, p0 = r0;  a_i = (r_i,r_i)/(p_i,A p_i);  x += a_i p_i;
+//     r_{i+1} = r_i - a_i A p_i;  b_i = (r_{i+1},r_{i+1})/(r_i,r_i);  p_{i+1} = r_{i+1} + b_i p_i.
+// Every r_i, p_i and x - g lies in the Krylov space of r0, so with a basis
+//     v_0 = r0,  v_1 = tau(A) v_0,  v_{k+1} = 2 tau(A) v_k - v_{k-1},
+//     tau(A) = (2/lmax) A - I,  lmax = 2 sum_d 1/h_d^2 >= spectrum of A
+// (Chebyshev polynomials on the spectrum, so the basis stays well
+// conditioned) the recurrence runs exactly in coefficient space: A acts as a
+// tridiagonal matrix T on coefficients and every inner product is c^T G c'
+// with G_jk = (v_j, v_k).  Pass 1 streams g once and accumulates the
+// (m+1)x(m+1) Gram matrix per band; one thread per band then replays the
+// reference's steps (same a_i, b_i, same early exit rr < 1e-24); pass 2
+// streams g once more and writes Pg = g + sum_k d_k v_k.  Two reads and one
+// write of psi-sized data replace ~30 passes of the step-by-step form.
+//
+// Both passes march z with the levels of the basis pipelined in registers:
+// at step s level l (l = 0 is g, l = k+1 is v_k) has its newest plane at
+// z = s - l; level l+1 at plane n uses level l at planes n-1, n, n+1 (z from
+// registers, x and y from one LDS plane per level).  The y range shrinks by
+// one row per level (redundant halo rows), the z range by one plane.
+#include "common.h"
+
+#include <algorithm>
+
+namespace rsdft {
+
+__device__ __forceinline__ int wrapz(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+__device__ __forceinline__ long xcd_map(long L, long total) {
+    if ((total & 7) != 0) return L;
+    const long per = total >> 3;
+    return (L & 7) * per + (L >> 3);
+}
+
+// Queue depth of level l for NLEV levels above g: three planes for the next
+// level's stencil, and back to plane s - NLEV where all levels are combined.
+__host__ __device__ constexpr int qdepth(int l, int NLEV) {
+    return l == NLEV ? 1 : ((NLEV - l + 1) > 3 ? (NLEV - l + 1) : 3);
+}
+__host__ __device__ constexpr int qoff(int l, int NLEV) {
+    return l == 0 ? 0 : qoff(l - 1, NLEV) + qdepth(l - 1, NLEV);
+}
+// slot of level l holding plane s - NLEV (the plane where all levels meet)
+__host__ __device__ constexpr int qout(int l, int NLEV) {
+    return qoff(l, NLEV) + qdepth(l, NLEV) - 1 - (NLEV - l);
+}
+
+// MODE 0: Gram pass, NLEV = m + 1, part[(j*(NB)+b)*tiles + tile] over the
+//         (m+1)(m+2)/2 entries j <= k of G (row-major upper triangle).
+// MODE 1: apply pass, NLEV = m, PG = g + sum_{k<m} d[b][k] v_k.
+template <class T, int PX, int NLEV, int MODE>
+__global__ void __launch_bounds__(1024)
+k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2, int N3,
+           int ty, int tz, int tpr, double c0, double c1, double c2, double c3, double s1,
+           int nb, const int *__restrict__ active, const double *__restrict__ dcoef,
+           double *__restrict__ part, int tiles)
+{
+    constexpr int NG = (NLEV) * (NLEV + 1) / 2;       // Gram entries (levels 1..NLEV)
+    constexpr int QTOT = qoff(NLEV, NLEV) + 1;
+    extern __shared__ unsigned char smem_raw[];
+    T *Lbase = reinterpret_cast<T *>(smem_raw);
+    __shared__ double red[16][MODE == 0 ? NG : 1];
+
+    const long total = (long)tiles * nb;
+    const long w = xcd_map(blockIdx.x, total);
+    if (w >= total) return;
+    const int band = (int)(w / tiles);
+    const int tile = (int)(w - (long)band * tiles);
+    if (!active[band]) return;
+    const int nty = N2 / ty;
+    const int y0 = (tile % nty) * ty;
+    const int z0 = (tile / nty) * tz;
+    const int rows = ty + 2 * NLEV;
+    const int W = N1 + 2;
+    const int tid = threadIdx.x;
+    const int r = tid / tpr;                 // 0 .. rows-1
+    const int cx0 = tid - r * tpr;
+    const int gy = wrapz(y0 - NLEV + r, N2);
+    const long plane = (long)N1 * N2;
+    const T *src = G + (long)band * ML;
+    const long rowoff = (long)gy * N1 + cx0;
+    const bool interior = r >= NLEV && r < NLEV + ty;
+
+    double dk[NLEV > 0 ? NLEV : 1];
+    if (MODE == 1) {
+#pragma unroll
+        for (int k = 0; k < NLEV; ++k) dk[k] = dcoef[(long)band * NLEV + k];
+    }
+    double gacc[MODE == 0 ? NG : 1];
+#pragma unroll
+    for (int k = 0; k < (MODE == 0 ? NG : 1); ++k) gacc[k] = 0.0;
+
+    T q[QTOT][PX];
+#pragma unroll
+    for (int k = 0; k < QTOT; ++k)
+#pragma unroll
+        for (int px = 0; px < PX; ++px) q[k][px] = T(0);
+
+    const T tc0 = (T)c0, tc1 = (T)c1, tc2 = (T)c2, tc3 = (T)c3, ts1 = (T)s1;
+    for (int s = z0 - NLEV; s < z0 + tz + NLEV; ++s) {
+        // level 0: newest plane s of g
+        {
+            const T *p = src + (long)wrapz(s, N3) * plane + rowoff;
+#pragma unroll
+            for (int px = 0; px < PX; ++px) q[qoff(0, NLEV) + qdepth(0, NLEV) - 1][px] = p[px * tpr];
+        }
+#pragma unroll
+        for (int l = 0; l < NLEV; ++l) {
+            // stage level l at plane s-l-1 (queue slot depth-2) into LDS plane l
+            T *Lp = Lbase + (long)l * rows * W;
+            const int qs = qoff(l, NLEV) + qdepth(l, NLEV) - 2;
+            if (r >= l && r < rows - l) {
+#pragma unroll
+                for (int px = 0; px < PX; ++px) {
+                    const int x = cx0 + px * tpr;
+                    Lp[r * W + 1 + x] = q[qs][px];
+                    if (x == 0) Lp[r * W + 1 + N1] = q[qs][px];
+                    if (x == N1 - 1) Lp[r * W] = q[qs][px];
+                }
+            }
+            __syncthreads();
+            // level l+1 at plane s-l-1 on rows [l+1, rows-l-1)
+            const int qn = qoff(l + 1, NLEV) + qdepth(l + 1, NLEV) - 1;   // newest slot of l+1
+            if (r >= l + 1 && r < rows - l - 1) {
+#pragma unroll
+                for (int px = 0; px < PX; ++px) {
+                    const int x = cx0 + px * tpr;
+                    const T *lr = Lp + r * W + 1 + x;
+                    const T cen = q[qs][px];
+                    T a = tc0 * cen;
+                    a = a + tc1 * (lr[1] + lr[-1]) + tc2 * (lr[W] + lr[-W]) +
+                        tc3 * (q[qs + 1][px] + q[qs - 1][px]);
+                    T v;
+                    if (l == 0) {
+                        v = cen - a;                          // v_0 = g - A g
+                    } else if (l == 1) {
+                        v = ts1 * a - cen;                    // v_1 = tau(A) v_0
+                    } else {
+                        // v_l = 2 tau(A) v_{l-1} - v_{l-2}; v_{l-2} is level l-1 at this plane
+                        const T prev = q[qoff(l - 1, NLEV) + qdepth(l - 1, NLEV) - 3][px];
+                        v = T(2) * (ts1 * a - cen) - prev;
+                    }
+                    q[qn][px] = v;
+                }
+            }
+        }
+        // all levels at plane zo = s - NLEV
+        const int zo = s - NLEV;
+        if (interior && zo >= z0) {
+            if (MODE == 0) {
+                T vv[NLEV][PX];
+#pragma unroll
+                for (int l = 1; l <= NLEV; ++l)
+#pragma unroll
+                    for (int px = 0; px < PX; ++px) vv[l - 1][px] = q[qout(l, NLEV)][px];
+#pragma unroll
+                for (int px = 0; px < PX; ++px) {
+                    int e = 0;
+#pragma unroll
+                    for (int j = 0; j < NLEV; ++j)
+#pragma unroll
+                        for (int k = j; k < NLEV; ++k)
+                            gacc[e++] += (double)vv[j][px] * (double)vv[k][px];
+                }
+            } else {
+                T *dst = PG + (long)band * ML + (long)zo * plane + rowoff;
+#pragma unroll
+                for (int px = 0; px < PX; ++px) {
+                    double x = (double)q[qout(0, NLEV)][px];
+#pragma unroll
+                    for (int k = 0; k < NLEV; ++k) x += dk[k] * (double)q[qout(k + 1, NLEV)][px];
+                    dst[px * tpr] = (T)x;
+                }
+            }
+        }
+        // shift every queue by one plane
+#pragma unroll
+        for (int l = 0; l <= NLEV; ++l)
+#pragma unroll
+            for (int k = 0; k < qdepth(l, NLEV) - 1; ++k)
+#pragma unroll
+                for (int px = 0; px < PX; ++px) q[qoff(l, NLEV) + k][px] = q[qoff(l, NLEV) + k + 1][px];
+        if (NLEV == 1) __syncthreads();
+    }
+    if (MODE == 0) {
+        const int lane = tid & 63, wave = tid >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+        for (int e = 0; e < NG; ++e) {
+            double v = gacc[e];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) red[wave][e] = v;
+        }
+        __syncthreads();
+        if (tid < NG) {
+            double t = 0.0;
+            for (int k = 0; k < nw; ++k) t += red[k][tid];
+            part[((long)tid * nb + band) * tiles + tile] = t;
+        }
+    }
+}
+
+// Per band: Gram matrix -> the reference's mloop CG steps in coefficient
+// space -> d[b][0..m-1] (Pg = g + sum d_k v_k); pc_active[b] = 0 when
+// (r0,r0) < 1e-24 (the reference's early return), which gives d = 0.
+template <int M>
+__global__ void k_pc_sstep_scalars(const double *__restrict__ part, int nb, int tiles, double lmax,
+                                   const int *__restrict__ active, double *__restrict__ dcoef)
+{
+    constexpr int N = M + 1;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb || !active[b]) return;
+    double Gm[N][N];
+    int e = 0;
+    for (int j = 0; j < N; ++j)
+        for (int k = j; k < N; ++k) {
+            double t = 0.0;
+            const double *p = part + ((long)e * nb + b) * tiles;
+            for (int c = 0; c < tiles; ++c) t += p[c];
+            Gm[j][k] = t;
+            Gm[k][j] = t;
+            ++e;
+        }
+    double d[M];
+    for (int k = 0; k < M; ++k) d[k] = 0.0;
+    double rr1 = Gm[0][0];
+    if (!(rr1 < 1.0e-24)) {
+        double rc[N], pc[N], qc[N];
+        for (int k = 0; k < N; ++k) rc[k] = pc[k] = 0.0;
+        rc[0] = pc[0] = 1.0;
+        const double h = 0.5 * lmax, qh = 0.25 * lmax;
+        for (int il = 1; il <= M; ++il) {
+            // q = T p  (A in the Chebyshev basis)
+            for (int j = 0; j < N; ++j) qc[j] = 0.0;
+            for (int k = 0; k < N; ++k) {
+                if (pc[k] == 0.0) continue;
+                qc[k] += h * pc[k];
+                if (k == 0) {
+                    if (N > 1) qc[1] += h * pc[0];
+                } else {
+                    qc[k - 1] += qh * pc[k];
+                    if (k + 1 < N) qc[k + 1] += qh * pc[k];
+                }
+            }
+            double pAp = 0.0;
+            for (int j = 0; j < N; ++j)
+                for (int k = 0; k < N; ++k) pAp += pc[j] * Gm[j][k] * qc[k];
+            const double rr0 = rr1;
+            const double a = rr0 / pAp;
+            for (int k = 0; k < M; ++k) d[k] += a * pc[k];
+            if (il == M) break;
+            for (int k = 0; k < N; ++k) rc[k] -= a * qc[k];
+            rr1 = 0.0;
+            for (int j = 0; j < N; ++j)
+                for (int k = 0; k < N; ++k) rr1 += rc[j] * Gm[j][k] * rc[k];
+            const double bb = rr1 / rr0;
+            for (int k = 0; k < N; ++k) pc[k] = rc[k] + bb * pc[k];
+        }
+    }
+    for (int k = 0; k < M; ++k) dcoef[(long)b * M + k] = d[k];
+}
+
+struct SstepGeom {
+    int ty, tz, tiles, threads, px;
+};
+
+static SstepGeom sstep_geom(const GridDesc &g, int nlev, int px)
+{
+    SstepGeom G{};
+    G.px = px;
+    const int tpr = g.N1 / px;
+    G.ty = 0;
+    for (int ty = 1; ty <= g.N2; ++ty) {
+        if (g.N2 % ty) continue;
+        const int thr = tpr * (ty + 2 * nlev);
+        if (thr > 1024) break;
+        if (ty >= 8 || ty == g.N2) { G.ty = ty; G.threads = thr; if (thr >= 384) break; }
+    }
+    if (G.ty == 0) {
+        for (int ty = g.N2; ty >= 1; --ty)
+            if (g.N2 % ty == 0 && tpr * (ty + 2 * nlev) <= 1024) { G.ty = ty; G.threads = tpr * (ty + 2 * nlev); break; }
+    }
+    int tz = g.N3;
+    for (int c = 1; c <= g.N3; ++c)
+        if (g.N3 % c == 0 && g.N3 / c <= 24) { tz = g.N3 / c; break; }
+    G.tz = tz;
+    G.tiles = (g.N2 / G.ty) * (g.N3 / G.tz);
+    return G;
+}
+
+template <class T, int M>
+static int sstep_launch(const T *g, T *pg, int nb, const int *active, double *part, double *dcoef)
+{
+    Context &c = ctx();
+    const GridDesc &gd = c.g;
+    constexpr int PX = sizeof(T) == 8 ? 2 : 4;
+    const double hx = 1.0 / (gd.h[0] * gd.h[0]), hy = 1.0 / (gd.h[1] * gd.h[1]),
+                 hz = 1.0 / (gd.h[2] * gd.h[2]);
+    const double c0 = hx + hy + hz, c1 = -0.5 * hx, c2 = -0.5 * hy, c3 = -0.5 * hz;
+    const double lmax = 2.0 * c0;
+    const double s1 = 2.0 / lmax;
+    // pass 1: Gram of v_0 .. v_M
+    {
+        const SstepGeom G = sstep_geom(gd, M + 1, PX);
+        if (G.ty == 0) return set_error(kErrArg, "no s-step tiling");
+        const size_t lds = (size_t)(M + 1) * (G.ty + 2 * (M + 1)) * (gd.N1 + 2) * sizeof(T);
+        const long nblk = (long)G.tiles * nb;
+        hipLaunchKernelGGL((k_pc_sstep<T, PX, M + 1, 0>), dim3((nblk + 7) / 8 * 8), dim3(G.threads),
+                           lds, c.stream, g, pg, gd.ML, gd.N1, gd.N2, gd.N3, G.ty, G.tz, gd.N1 / PX,
+                           c0, c1, c2, c3, s1, nb, active, (const double *)nullptr, part, G.tiles);
+        RS_HIP(hipGetLastError());
+        hipLaunchKernelGGL((k_pc_sstep_scalars<M>), dim3((nb + 127) / 128), dim3(128), 0, c.stream,
+                           part, nb, G.tiles, lmax, active, dcoef);
+        RS_HIP(hipGetLastError());
+    }
+    // pass 2: Pg = g + sum d_k v_k
+    {
+        const SstepGeom G = sstep_geom(gd, M, PX);
+        const size_t lds = (size_t)M * (G.ty + 2 * M) * (gd.N1 + 2) * sizeof(T);
+        const long nblk = (long)G.tiles * nb;
+        hipLaunchKernelGGL((k_pc_sstep<T, PX, M, 1>), dim3((nblk + 7) / 8 * 8), dim3(G.threads),
+                           lds, c.stream, g, pg, gd.ML, gd.N1, gd.N2, gd.N3, G.ty, G.tz, gd.N1 / PX,
+                           c0, c1, c2, c3, s1, nb, active, (const double *)dcoef, part, G.tiles);
+        RS_HIP(hipGetLastError());
+    }
+    return kOK;
+}
+
+size_t precond_sstep_part_elems(int nb)
+{
+    Context &c = ctx();
+    size_t mx = 0;
+    for (int m = 1; m <= 4; ++m) {
+        const SstepGeom G = sstep_geom(c.g, m + 1, 2);
+        const SstepGeom G4 = sstep_geom(c.g, m + 1, 4);
+        const size_t t = (size_t)std::max(G.tiles, G4.tiles);
+        mx = std::max(mx, t * (size_t)((m + 1) * (m + 2) / 2) * nb);
+    }
+    return mx + (size_t)4 * nb;
+}
+
+template <class T>
+int precond_sstep(const T *g, T *pg, int nb, const int *active, double *part, double *dcoef)
+{
+    switch (ctx().mloop) {
+    case 1: return sstep_launch<T, 1>(g, pg, nb, active, part, dcoef);
+    case 2: return sstep_launch<T, 2>(g, pg, nb, active, part, dcoef);
+    case 3: return sstep_launch<T, 3>(g, pg, nb, active, part, dcoef);
+    case 4: return sstep_launch<T, 4>(g, pg, nb, active, part, dcoef);
+    default: return set_error(kErrArg, "s-step preconditioner built for mloop 1..4");
+    }
+}
+
+template int precond_sstep<double>(const double *, double *, int, const int *, double *, double *);
+template int precond_sstep<float>(const float *, float *, int, const int *, double *, double *);
+
+}  // namespace rsdft

@@ -31,7 +31,7 @@ PREC_MIXED_BF16 = 2
 EXPORTS = (
     "rsdft_gpu_init", "rsdft_gpu_set_vloc", "rsdft_gpu_set_projectors",
     "rsdft_gpu_set_projector_derivatives", "rsdft_gpu_set_precision",
-    "rsdft_gpu_set_solver_params", "rsdft_gpu_hamiltonian", "rsdft_gpu_op_kinetic",
+    "rsdft_gpu_set_solver_params", "rsdft_gpu_set_precond_mode", "rsdft_gpu_hamiltonian", "rsdft_gpu_op_kinetic",
     "rsdft_gpu_op_localpot", "rsdft_gpu_op_nonlocal", "rsdft_gpu_wf_attach",
     "rsdft_gpu_wf_sync", "rsdft_gpu_wf_detach", "rsdft_gpu_conjugate_gradient",
     "rsdft_gpu_gram_schmidt", "rsdft_gpu_subspace_diag", "rsdft_gpu_sweep",
@@ -64,6 +64,7 @@ def load_library(path: str = LIB_PATH):
         "rsdft_gpu_set_projector_derivatives": [I, P, I, P, P, P],
         "rsdft_gpu_set_precision": [I],
         "rsdft_gpu_set_solver_params": [I, I],
+        "rsdft_gpu_set_precond_mode": [I],
         "rsdft_gpu_hamiltonian": [I, I, P, P, I, I, I, I],
         "rsdft_gpu_op_kinetic": [I, P, P, I, I, I, I],
         "rsdft_gpu_op_localpot": [I, I, I, P, P],
@@ -137,6 +138,10 @@ class RsdftGpu:
     def set_precision(self, mode: int):
         self._call("rsdft_gpu_set_precision", int(mode))
         self.precision = int(mode)
+
+    def set_precond_mode(self, mode: int):
+        """1: s-step preconditioner (default), 0: step-by-step recurrence."""
+        self._call("rsdft_gpu_set_precond_mode", int(mode))
 
     def set_vloc(self, vloc):
         v = _f64(vloc)

@@ -71,19 +71,24 @@ def test_subspace_diag_fp64(dev, oracle_si8, si8):
     assert rel(x * s[:, None], xo) < 1e-8
 
 
-def test_conjugate_gradient_fp64(dev, oracle_si8, si8):
-    print("case: conjugate_gradient fp64 si8", flush=True)
+@pytest.mark.parametrize("pc_mode", [1, 0])
+def test_conjugate_gradient_fp64(dev, oracle_si8, si8, pc_mode):
+    print(f"case: conjugate_gradient fp64 si8 precond mode {pc_mode}", flush=True)
     q = oracle_si8.gram_schmidt(si8.psi0())
     q, _, _ = oracle_si8.subspace_diag(q)
+    dev.set_precond_mode(pc_mode)
     x, e, r = dev.conjugate_gradient(q)
+    dev.set_precond_mode(1)
     xo, eo, ro = oracle_si8.conjugate_gradient(q)
     assert np.max(np.abs(e - eo)) < 1e-10
     assert np.max(np.abs(r - ro) / np.maximum(ro, 1e-30)) < 1e-8
     assert rel(x, xo) < 1e-9
 
 
-def test_four_sweeps_fp64(dev, oracle_si8, si8):
-    print("case: 4 sweeps fp64 si8", flush=True)
+@pytest.mark.parametrize("pc_mode", [1, 0])
+def test_four_sweeps_fp64(dev, oracle_si8, si8, pc_mode):
+    print(f"case: 4 sweeps fp64 si8 precond mode {pc_mode}", flush=True)
+    dev.set_precond_mode(pc_mode)
     q = oracle_si8.gram_schmidt(si8.psi0())
     x, eo = q, None
     xg = q.copy()
@@ -92,6 +97,7 @@ def test_four_sweeps_fp64(dev, oracle_si8, si8):
         xg, eg, rg = dev.sweep(xg, 1)
         print(f"  sweep {it}: max|de| = {np.max(np.abs(eg - eo)):.3e}", flush=True)
         assert np.max(np.abs(eg - eo)) < 1e-10
+    dev.set_precond_mode(1)
 
 
 def test_density_and_force_fp64(dev, oracle_si8, si8):
