@@ -188,8 +188,10 @@ def main():
             "hpsi_ms_fp32": float(hpm[0]),
             "hpsi_ms_fp64": float(hp64[0]),
             "hpsi_stencil_gbs_fp64": ach64,
-            "phase_ms_mixed": {k: v / K for k, v in timm.items() if k != "hpsi_bands"},
-            "phase_ms_fp64": {k: v / K for k, v in tim64.items() if k != "hpsi_bands"},
+            "phase_ms_mixed": {k: v / K for k, v in timm.items() if k not in ("hpsi_bands", "eig_refined")},
+            "eig_refined_mixed": timm["eig_refined"],
+            "phase_ms_fp64": {k: v / K for k, v in tim64.items() if k not in ("hpsi_bands", "eig_refined")},
+            "eig_refined_fp64": tim64["eig_refined"],
         },
         "roofline": {
             "bound": "hbm",

@@ -70,6 +70,12 @@ RSDFT_API int rsdft_gpu_set_solver_params(int ncg, int mloop);
  * (mloop <= 4); 0 -- the step-by-step recurrence. */
 RSDFT_API int rsdft_gpu_set_precond_mode(int mode);
 
+/* Eigensolver of the subspace_diag step (subspace_diag_la_module.f90:159-169):
+ * 1 (default) quadratically convergent refinement from the identity when
+ * Hsub is nearly diagonal, falling back to rocsolver_dsyevd otherwise;
+ * 0 -- always rocsolver_dsyevd. */
+RSDFT_API int rsdft_gpu_set_eigensolver(int mode);
+
 /* hamiltonian(k,s,tpsi,htpsi,n1,n2,ib1,ib2) (hamiltonian_module.f90:20-72):
  * htpsi = T tpsi + Vloc tpsi + V_NL tpsi (htpsi overwritten).  Gamma only:
  * k must be 1, s must be 1. */
@@ -139,7 +145,8 @@ RSDFT_API int rsdft_gpu_test_hpsi(int nrep, double *times_ms);
 /* Per-phase timers in milliseconds since the last reset (watch_module.f90
  * time_hmlt/time_cgpc analogue): [0] hamiltonian, [1] preconditioner,
  * [2] CG total, [3] Gram-Schmidt, [4] subspace_diag, [5] sweep total,
- * [6] number of band-hamiltonian applications.  n <= 8. */
+ * [6] number of band-hamiltonian applications, [7] number of subspace
+ * eigensolves done by refinement.  n <= 8. */
 RSDFT_API int rsdft_gpu_get_timers(double *t, int n);
 RSDFT_API int rsdft_gpu_reset_timers(void);
 

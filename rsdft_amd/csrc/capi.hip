@@ -401,6 +401,13 @@ int rsdft_gpu_set_precond_mode(int mode)
     return kOK;
 }
 
+int rsdft_gpu_set_eigensolver(int mode)
+{
+    if (mode < 0 || mode > 1) return set_error(kErrArg, "eigensolver mode %d", mode);
+    ctx().eig_mode = mode;
+    return kOK;
+}
+
 int rsdft_gpu_set_solver_params(int ncg, int mloop)
 {
     if (ncg < 0 || mloop < 0) return set_error(kErrArg, "ncg=%d mloop=%d", ncg, mloop);
@@ -695,7 +702,7 @@ int rsdft_gpu_get_timers(double *t, int n)
     if (n > 8) n = 8;
     for (int k = 0; k < n && k < 6; ++k) t[k] = c.timers.ms[k];
     if (n > 6) t[6] = (double)c.hpsi_bands;
-    if (n > 7) t[7] = 0.0;
+    if (n > 7) t[7] = (double)c.eig_refined;
     return kOK;
 }
 
@@ -704,6 +711,7 @@ int rsdft_gpu_reset_timers(void)
     Context &c = ctx();
     for (double &x : c.timers.ms) x = 0.0;
     c.hpsi_bands = 0;
+    c.eig_refined = 0;
     return kOK;
 }
 
@@ -732,6 +740,8 @@ int rsdft_gpu_finalize(void)
     c.nl.nz = 0;
     c.nl.natom = 0;
     c.pc_mode = 1;
+    c.eig_mode = 1;
+    c.eig_refined = 0;
     for (double &x : c.timers.ms) x = 0.0;
     c.hpsi_bands = 0;
     return kOK;

@@ -129,6 +129,8 @@ struct Context {
     int prec = 0;
     int ncg = 3, mloop = 3;
     int pc_mode = 1;   // 0: step-by-step preconditioner recurrence, 1: s-step form
+    int eig_mode = 1;  // 0: always dsyevd, 1: refinement when Hsub is nearly diagonal
+    long eig_refined = 0;
 
     DevMem vloc64, vloc32;
     NonlocalTables nl;

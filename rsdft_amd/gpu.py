@@ -31,7 +31,8 @@ PREC_MIXED_BF16 = 2
 EXPORTS = (
     "rsdft_gpu_init", "rsdft_gpu_set_vloc", "rsdft_gpu_set_projectors",
     "rsdft_gpu_set_projector_derivatives", "rsdft_gpu_set_precision",
-    "rsdft_gpu_set_solver_params", "rsdft_gpu_set_precond_mode", "rsdft_gpu_hamiltonian", "rsdft_gpu_op_kinetic",
+    "rsdft_gpu_set_solver_params", "rsdft_gpu_set_precond_mode", "rsdft_gpu_set_eigensolver",
+    "rsdft_gpu_hamiltonian", "rsdft_gpu_op_kinetic",
     "rsdft_gpu_op_localpot", "rsdft_gpu_op_nonlocal", "rsdft_gpu_wf_attach",
     "rsdft_gpu_wf_sync", "rsdft_gpu_wf_detach", "rsdft_gpu_conjugate_gradient",
     "rsdft_gpu_gram_schmidt", "rsdft_gpu_subspace_diag", "rsdft_gpu_sweep",
@@ -65,6 +66,7 @@ def load_library(path: str = LIB_PATH):
         "rsdft_gpu_set_precision": [I],
         "rsdft_gpu_set_solver_params": [I, I],
         "rsdft_gpu_set_precond_mode": [I],
+        "rsdft_gpu_set_eigensolver": [I],
         "rsdft_gpu_hamiltonian": [I, I, P, P, I, I, I, I],
         "rsdft_gpu_op_kinetic": [I, P, P, I, I, I, I],
         "rsdft_gpu_op_localpot": [I, I, I, P, P],
@@ -142,6 +144,10 @@ class RsdftGpu:
     def set_precond_mode(self, mode: int):
         """1: s-step preconditioner (default), 0: step-by-step recurrence."""
         self._call("rsdft_gpu_set_precond_mode", int(mode))
+
+    def set_eigensolver(self, mode: int):
+        """1: refinement when Hsub is nearly diagonal (default), 0: always dsyevd."""
+        self._call("rsdft_gpu_set_eigensolver", int(mode))
 
     def set_vloc(self, vloc):
         v = _f64(vloc)
@@ -268,7 +274,7 @@ class RsdftGpu:
         t = np.zeros(8)
         self._call("rsdft_gpu_get_timers", _p(t), 8)
         return {"hamiltonian": t[0], "precond": t[1], "cg": t[2], "gram_schmidt": t[3],
-                "subspace_diag": t[4], "sweep": t[5], "hpsi_bands": t[6]}
+                "subspace_diag": t[4], "sweep": t[5], "hpsi_bands": t[6], "eig_refined": t[7]}
 
     def reset_timers(self):
         self._call("rsdft_gpu_reset_timers")

@@ -108,3 +108,26 @@ def test_density_and_force_fp64(dev, oracle_si8, si8):
     f = dev.calc_force_ps_nloc2(q, si8.occ)
     fo = oracle_si8.force_nloc(q, si8.occ)
     assert rel(f, fo) < 1e-11
+
+
+def test_refinement_eigensolver_matches_dsyevd(dev, oracle_si8, si8):
+    """Nearly diagonal Hsub (Ritz vectors after 6 sweeps, one CG + GS): the
+    refinement path and rocsolver_dsyevd give the same Ritz pairs."""
+    print("case: refinement eigensolver vs dsyevd si8", flush=True)
+    x = oracle_si8.gram_schmidt(si8.psi0())
+    for _ in range(6):
+        x, _, _ = oracle_si8.sweep(x)
+    x, _, _ = oracle_si8.conjugate_gradient(x)
+    q = oracle_si8.gram_schmidt(x)
+    xo, eo, _ = oracle_si8.subspace_diag(q)
+    dev.reset_timers()
+    dev.set_eigensolver(1)
+    x1, e1 = dev.subspace_diag(q)
+    used = dev.timers()["eig_refined"]
+    dev.set_eigensolver(0)
+    x0, e0 = dev.subspace_diag(q)
+    dev.set_eigensolver(1)
+    assert used == 1
+    assert np.max(np.abs(e1 - eo)) < 1e-10 and np.max(np.abs(e0 - eo)) < 1e-10
+    s = np.sign(np.sum(x1 * xo, axis=1))
+    assert rel(x1 * s[:, None], xo) < 1e-8
