@@ -224,8 +224,9 @@ int rsdft_gpu_init(int N1, int N2, int N3, const double *Hgrid, double dV, int M
     c.g.coef_lap0 = coef_lap0;
     for (int m = 0; m < Md; ++m)
         for (int d = 0; d < 3; ++d) c.g.coef_lap[d][m] = coef_lap[d + 3 * m];
-    c.geom_h = choose_geom(N1, N2, N3, Md, 2, 8);
-    c.geom_h32 = choose_geom(N1, N2, N3, Md, 4, 4);
+    c.geom_h = choose_hpsi_geom(N1, N2, N3, Md, kHpsiPx64, Md, 8, kHpsiThreads64);
+    c.geom_h32 = choose_hpsi_geom(N1, N2, N3, Md, kHpsiPx32, (Md + kHpsiPx32 - 1) / kHpsiPx32 * kHpsiPx32,
+                                  4, kHpsiThreads32);
     c.geom_pc = choose_geom(N1, N2, N3, 1, 4, 8);
     if (c.geom_h.threads == 0 || c.geom_h32.threads == 0 || c.geom_pc.threads == 0)
         return set_error(kErrArg, "no stencil tiling for grid %dx%dx%d", N1, N2, N3);
@@ -256,78 +257,133 @@ int rsdft_gpu_set_projectors(int nzlma, int maxmjj, const int *MJJ, const int *J
     Context &c = ctx();
     NonlocalTables &nl = c.nl;
     const long ML = c.g.ML;
-    nl.nz = nzlma;
-    if (nzlma <= 0) { nl.nnz = 0; return kOK; }
-    std::vector<int> rptr(nzlma + 1, 0);
+    nl.nz = 0;
+    nl.ng = 0;
+    if (nzlma <= 0) return kOK;
+    // per projector: (grid point, uV) sorted by point, repeated points merged
+    // (a sphere larger than the cell lists a point once per image)
+    using Entry = std::pair<int, double>;
+    std::vector<std::vector<Entry>> pts(nzlma);
     for (int l = 0; l < nzlma; ++l) {
         if (MJJ[l] < 0 || MJJ[l] > maxmjj) return set_error(kErrArg, "MJJ(%d)=%d", l + 1, MJJ[l]);
-        rptr[l + 1] = rptr[l] + MJJ[l];
-    }
-    const long nnz = rptr[nzlma];
-    std::vector<int> ridx(nnz);
-    std::vector<double> rval(nnz);
-    std::vector<int> cptr(ML + 1, 0);
-    for (int l = 0; l < nzlma; ++l)
+        auto &P = pts[l];
+        P.reserve(MJJ[l]);
         for (int j = 0; j < MJJ[l]; ++j) {
             const int gi = JJP[(long)l * maxmjj + j] - 1;
-            if (gi < 0 || gi >= ML) return set_error(kErrArg, "JJP(%d,%d)=%d out of grid", j + 1, l + 1, gi + 1);
-            ridx[rptr[l] + j] = gi;
-            rval[rptr[l] + j] = uVk[(long)l * maxmjj + j];
-            cptr[gi + 1]++;
+            if (gi < 0 || gi >= ML)
+                return set_error(kErrArg, "JJP(%d,%d)=%d out of grid", j + 1, l + 1, gi + 1);
+            P.push_back({gi, uVk[(long)l * maxmjj + j]});
         }
-    for (long i = 0; i < ML; ++i) cptr[i + 1] += cptr[i];
-    std::vector<int> fill(cptr.begin(), cptr.end() - 1), clma(nnz);
-    std::vector<double> cval(nnz);
-    // column entries in lma order per point: the reference adds projector
-    // contributions in lma order (ps_nloc2_op_module.f90:255-263)
-    for (int l = 0; l < nzlma; ++l)
-        for (int j = 0; j < MJJ[l]; ++j) {
-            const int gi = ridx[rptr[l] + j];
-            const int e = fill[gi]++;
-            clma[e] = l;
-            cval[e] = rval[rptr[l] + j];
+        std::stable_sort(P.begin(), P.end(),
+                         [](const Entry &a, const Entry &b) { return a.first < b.first; });
+        size_t w = 0;
+        for (size_t k = 0; k < P.size(); ++k) {
+            if (w > 0 && P[w - 1].first == P[k].first) P[w - 1].second += P[k].second;
+            else P[w++] = P[k];
         }
-    // covered points (inside some sphere) and their entry ranges
-    std::vector<int> cpts, cvptr;
-    for (long i = 0; i < ML; ++i)
-        if (cptr[i + 1] > cptr[i]) { cpts.push_back((int)i); cvptr.push_back(cptr[i]); }
-    cvptr.push_back((int)nnz);
-    const int ncov = (int)cpts.size();
-    int maxblk = 0;
-    for (int p0 = 0; p0 < ncov; p0 += 256) {
-        const int p1 = std::min(ncov, p0 + 256);
-        maxblk = std::max(maxblk, cvptr[p1] - cvptr[p0]);
+        P.resize(w);
     }
-    std::vector<float> rval32(rval.begin(), rval.end()), cval32(cval.begin(), cval.end());
+    // groups: consecutive projectors whose point sets overlap by >= 3/4
+    std::vector<std::vector<int>> gmem, gset;
+    for (int l = 0; l < nzlma; ++l) {
+        if (!gmem.empty() && (int)gmem.back().size() < kGroupMax) {
+            const std::vector<int> &S = gset.back();
+            const auto &P = pts[l];
+            size_t a = 0, b = 0, inter = 0;
+            while (a < S.size() && b < P.size()) {
+                if (S[a] < P[b].first) ++a;
+                else if (S[a] > P[b].first) ++b;
+                else { ++inter; ++a; ++b; }
+            }
+            const size_t mx = std::max(S.size(), P.size());
+            if (mx > 0 && 4 * inter >= 3 * mx) {
+                std::vector<int> U;
+                U.reserve(S.size() + P.size() - inter);
+                a = 0; b = 0;
+                while (a < S.size() || b < P.size()) {
+                    if (b == P.size() || (a < S.size() && S[a] < P[b].first)) U.push_back(S[a++]);
+                    else if (a == S.size() || S[a] > P[b].first) U.push_back(P[b++].first);
+                    else { U.push_back(S[a]); ++a; ++b; }
+                }
+                gset.back().swap(U);
+                gmem.back().push_back(l);
+                continue;
+            }
+        }
+        gmem.push_back({l});
+        std::vector<int> S(pts[l].size());
+        for (size_t k = 0; k < S.size(); ++k) S[k] = pts[l][k].first;
+        gset.push_back(std::move(S));
+    }
+    const int ng = (int)gmem.size();
+    int gp = 1;
+    for (const auto &m : gmem)
+        while (gp < (int)m.size()) gp *= 2;
+    std::vector<int> gptr(ng + 1, 0), glma((size_t)ng * gp, -1);
+    for (int g = 0; g < ng; ++g) {
+        gptr[g + 1] = gptr[g] + (int)gset[g].size();
+        for (size_t s2 = 0; s2 < gmem[g].size(); ++s2) glma[(size_t)g * gp + s2] = gmem[g][s2];
+    }
+    const long npts = gptr[ng];
+    std::vector<int> gidx(npts);
+    std::vector<double> gval((size_t)npts * gp, 0.0);
+    for (int g = 0; g < ng; ++g) {
+        const auto &S = gset[g];
+        std::copy(S.begin(), S.end(), gidx.begin() + gptr[g]);
+        for (size_t s2 = 0; s2 < gmem[g].size(); ++s2) {
+            const auto &P = pts[gmem[g][s2]];
+            size_t a = 0;
+            for (const Entry &e : P) {
+                while (S[a] < e.first) ++a;
+                gval[((size_t)gptr[g] + a) * gp + s2] = e.second;
+            }
+        }
+    }
+    // greedy colouring: groups sharing a grid point get different colours
+    std::vector<uint32_t> pcol(ML, 0u);
+    std::vector<int> gcol(ng);
+    int ncol = 0;
+    for (int g = 0; g < ng; ++g) {
+        uint32_t used = 0;
+        for (int k = gptr[g]; k < gptr[g + 1]; ++k) used |= pcol[gidx[k]];
+        int col = 0;
+        while (col < 32 && (used >> col) & 1u) ++col;
+        if (col == 32) return set_error(kErrArg, "projector spheres overlap too densely");
+        gcol[g] = col;
+        ncol = std::max(ncol, col + 1);
+        for (int k = gptr[g]; k < gptr[g + 1]; ++k) pcol[gidx[k]] |= 1u << col;
+    }
+    std::vector<int> order;
+    order.reserve(ng);
+    nl.color_ptr.assign(1, 0);
+    for (int col = 0; col < ncol; ++col) {
+        for (int g = 0; g < ng; ++g)
+            if (gcol[g] == col) order.push_back(g);
+        nl.color_ptr.push_back((int)order.size());
+    }
+    std::vector<float> gval32(gval.begin(), gval.end());
     std::vector<double> scale(nzlma);
     for (int l = 0; l < nzlma; ++l) scale[l] = (double)iuV[l] * c.g.dV;
-    nl.nnz = nnz;
-    RS_TRY(nl.row_ptr.ensure(sizeof(int) * (nzlma + 1)));
-    RS_TRY(nl.row_idx.ensure(sizeof(int) * nnz));
-    RS_TRY(nl.row_val64.ensure(sizeof(double) * nnz));
-    RS_TRY(nl.row_val32.ensure(sizeof(float) * nnz));
-    RS_TRY(nl.col_ptr.ensure(sizeof(int) * (ML + 1)));
-    RS_TRY(nl.col_lma.ensure(sizeof(int) * nnz));
-    RS_TRY(nl.col_val64.ensure(sizeof(double) * nnz));
-    RS_TRY(nl.col_val32.ensure(sizeof(float) * nnz));
+    RS_TRY(nl.g_lma.ensure(sizeof(int) * glma.size()));
+    RS_TRY(nl.g_ptr.ensure(sizeof(int) * (ng + 1)));
+    RS_TRY(nl.g_idx.ensure(sizeof(int) * std::max<long>(npts, 1)));
+    RS_TRY(nl.g_val64.ensure(sizeof(double) * std::max<size_t>(gval.size(), 1)));
+    RS_TRY(nl.g_val32.ensure(sizeof(float) * std::max<size_t>(gval.size(), 1)));
+    RS_TRY(nl.g_order.ensure(sizeof(int) * ng));
     RS_TRY(nl.scale64.ensure(sizeof(double) * nzlma));
-    RS_HIP(hipMemcpy(nl.row_ptr.p, rptr.data(), sizeof(int) * (nzlma + 1), hipMemcpyHostToDevice));
-    RS_HIP(hipMemcpy(nl.row_idx.p, ridx.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-    RS_HIP(hipMemcpy(nl.row_val64.p, rval.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
-    RS_HIP(hipMemcpy(nl.row_val32.p, rval32.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
-    RS_HIP(hipMemcpy(nl.col_ptr.p, cptr.data(), sizeof(int) * (ML + 1), hipMemcpyHostToDevice));
-    RS_HIP(hipMemcpy(nl.col_lma.p, clma.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-    RS_HIP(hipMemcpy(nl.col_val64.p, cval.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
-    RS_HIP(hipMemcpy(nl.col_val32.p, cval32.data(), sizeof(float) * nnz, hipMemcpyHostToDevice));
+    RS_HIP(hipMemcpy(nl.g_lma.p, glma.data(), sizeof(int) * glma.size(), hipMemcpyHostToDevice));
+    RS_HIP(hipMemcpy(nl.g_ptr.p, gptr.data(), sizeof(int) * (ng + 1), hipMemcpyHostToDevice));
+    if (npts) {
+        RS_HIP(hipMemcpy(nl.g_idx.p, gidx.data(), sizeof(int) * npts, hipMemcpyHostToDevice));
+        RS_HIP(hipMemcpy(nl.g_val64.p, gval.data(), sizeof(double) * gval.size(), hipMemcpyHostToDevice));
+        RS_HIP(hipMemcpy(nl.g_val32.p, gval32.data(), sizeof(float) * gval.size(), hipMemcpyHostToDevice));
+    }
+    RS_HIP(hipMemcpy(nl.g_order.p, order.data(), sizeof(int) * ng, hipMemcpyHostToDevice));
     RS_HIP(hipMemcpy(nl.scale64.p, scale.data(), sizeof(double) * nzlma, hipMemcpyHostToDevice));
-    nl.ncov = ncov;
-    nl.max_blk_entries = maxblk;
-    if ((size_t)maxblk * (sizeof(int) + sizeof(double)) + 16 > 64 * 1024)
-        return set_error(kErrArg, "projector spheres too dense: %d entries per 256 points", maxblk);
-    RS_TRY(nl.cov_pts.ensure(sizeof(int) * std::max(ncov, 1)));
-    RS_TRY(nl.cov_ptr.ensure(sizeof(int) * (ncov + 1)));
-    if (ncov) RS_HIP(hipMemcpy(nl.cov_pts.p, cpts.data(), sizeof(int) * ncov, hipMemcpyHostToDevice));
-    RS_HIP(hipMemcpy(nl.cov_ptr.p, cvptr.data(), sizeof(int) * (ncov + 1), hipMemcpyHostToDevice));
+    nl.nz = nzlma;
+    nl.ng = ng;
+    nl.gp = gp;
+    nl.npts = npts;
     return kOK;
 }
 
@@ -723,9 +779,8 @@ int rsdft_gpu_finalize(void)
     DevMem *all[] = {&c.vloc64, &c.vloc32, &c.psi, &c.hx, &c.hp, &c.gk, &c.pgk, &c.pk, &c.rk,
                      &c.ppc, &c.qpc, &c.tmp, &c.coefC, &c.part, &c.stage, &c.mat, &c.mat2,
                      &c.mat3, &c.eval, &c.einfo, &c.evwork, &c.matT, &c.dots, &c.gemm_part, &c.opbuf, &c.pc_coef,
-                     &c.cg.buf, &c.cg.flags, &c.nl.row_ptr, &c.nl.row_idx, &c.nl.row_val64,
-                     &c.nl.row_val32, &c.nl.col_ptr, &c.nl.col_lma, &c.nl.col_val64,
-                     &c.nl.col_val32, &c.nl.scale64, &c.nl.d_ptr, &c.nl.d_idx, &c.nl.d_val, &c.nl.cov_pts, &c.nl.cov_ptr,
+                     &c.cg.buf, &c.cg.flags, &c.nl.g_lma, &c.nl.g_ptr, &c.nl.g_idx, &c.nl.g_val64,
+                     &c.nl.g_val32, &c.nl.g_order, &c.nl.scale64, &c.nl.d_ptr, &c.nl.d_idx, &c.nl.d_val,
                      &c.nl.amap_dev};
     for (DevMem *m : all) m->release();
     if (c.blas) rocblas_destroy_handle(c.blas);
@@ -738,6 +793,9 @@ int rsdft_gpu_finalize(void)
     c.wf_prec = -1;
     c.prec = 0;
     c.nl.nz = 0;
+    c.nl.ng = 0;
+    c.nl.npts = 0;
+    c.nl.color_ptr.clear();
     c.nl.natom = 0;
     c.pc_mode = 1;
     c.eig_mode = 1;

@@ -12,6 +12,10 @@
 namespace rsdft {
 
 constexpr int kMaxMd = 8;
+// hamiltonian stencil layouts: FP32 moves chunks of 2 consecutive x points per
+// thread, FP64 two x points tpr apart (strided); workgroup size limits
+constexpr int kHpsiPx32 = 2, kHpsiPx64 = 2;
+constexpr int kHpsiThreads32 = 1024, kHpsiThreads64 = 512;
 
 enum Status {
     kOK = 0,
@@ -78,15 +82,24 @@ struct GridDesc {
 // Sparse projector operator in two orientations.
 //   rows  : projector lma -> its grid points (CSR over lma), used to project
 //   cols  : grid point -> (lma, value) (CSR over grid points), used to apply
+// Kleinman-Bylander projector tables.  Projectors that share a sphere (the
+// s, p, ... channels of one atom: consecutive lma in the reference's order)
+// form a group with one union point list and gp value slots per point, so a
+// grid value is gathered once for all channels of its atom.  Groups of one
+// colour have disjoint point lists, so the apply can add to the output
+// without atomics, colour by colour.
+constexpr int kGroupMax = 16;
 struct NonlocalTables {
     int nz = 0;
-    long nnz = 0;
-    DevMem row_ptr, row_idx, row_val64, row_val32;
-    DevMem col_ptr, col_lma, col_val64, col_val32;
-    DevMem scale64;   // iuV(lma) * dV
-    int ncov = 0;             // grid points inside some sphere
-    int max_blk_entries = 0;  // max entries over blocks of 256 covered points
-    DevMem cov_pts, cov_ptr;  // covered points and their entry ranges in col_*
+    int ng = 0;        // groups
+    int gp = 1;        // value slots per point (power of two >= largest group)
+    long npts = 0;     // sum of union point counts
+    DevMem g_lma;      // [ng*gp] lma of each slot, -1 for an unused slot
+    DevMem g_ptr, g_idx;        // CSR over groups of the union points (0-based grid index)
+    DevMem g_val64, g_val32;    // [npts*gp] uV of each slot at each point (0 outside its sphere)
+    DevMem g_order;             // groups ordered by colour
+    std::vector<int> color_ptr; // colour k: g_order[color_ptr[k] .. color_ptr[k+1])
+    DevMem scale64;             // iuV(lma) * dV
     // force tables
     int natom = 0;
     long nnzd = 0;
@@ -97,11 +110,13 @@ struct NonlocalTables {
 
 // Stencil launch geometry chosen for the grid.
 struct StencilGeom {
-    int px = 4;   // consecutive x points per thread
+    int px = 4;   // x points per thread
     int ty = 1;   // y rows per workgroup
     int tz = 1;   // z planes per workgroup (z-march length)
     int threads = 64;
     int tiles = 1;  // (N2/ty)*(N3/tz)
+    int hm = 1;     // y-halo items per thread (ceil(2*Md/ty))
+    size_t lds = 0; // dynamic LDS bytes
 };
 
 // CG per-band scalars (FP64 on device).
@@ -160,6 +175,10 @@ Context &ctx();
 // ---- launchers (implemented in the .hip files) ----------------------------
 // geometry
 StencilGeom choose_geom(int N1, int N2, int N3, int md, int px, int elem_bytes);
+// Tile of the Md-order hamiltonian stencil: px points per thread, LDS row of
+// N1 + 2*pad elements, two LDS plane buffers.
+StencilGeom choose_hpsi_geom(int N1, int N2, int N3, int md, int px, int pad, int elem_bytes,
+                             int max_threads);
 
 // Hamiltonian pieces. flags bit0 kinetic, bit1 local, bit2 nonlocal, bit3 accumulate.
 enum { kKin = 1, kLoc = 2, kNl = 4, kAcc = 8 };

@@ -88,40 +88,36 @@ __global__ void k_check_info(const int *info, int *flag)
 //
 // After the first sweeps the Ritz vectors change little from sweep to sweep,
 // so Hsub (in the orthonormalised basis of the CG-updated Ritz vectors) is
-// nearly diagonal.  Starting from X = I + E0, E0_ij = H_ij/(H_jj - H_ii), the
-// refinement of Ogita and Aishima (Japan J. Indust. Appl. Math. 35, 2018)
-//     R = I - X^T X,  S = X^T H X,  l_i = S_ii/(1 - R_ii),
-//     E_ij = (S_ij + l_j R_ij)/(l_j - l_i)  (|l_i - l_j| > delta),  E_ii = R_ii/2,
-//     X <- X (I + E),  delta = 2(|S - diag(l)|_F + |H|_F |R|_F)
-// converges quadratically when the eigenvalues are separated by more than
-// delta.  Each step is four n x n FP64 GEMMs instead of ~1,500 small kernels
-// of the tridiagonal-reduction eigensolver.  Used only when every
-// off-diagonal element is below 0.3 of its diagonal gap; if a pair stays
-// inside delta or the iteration does not reach 1e-12 the eigensolve falls
-// back to rocsolver_dsyevd.
+// nearly diagonal, except for groups of nearly degenerate bands that CG mixes
+// strongly.  Starting from X = I, each iteration
+//   1. finds the clusters: connected components of the pairs whose
+//      first-order correction |E_ij| = |B_ij + l_j R_ij| / |l_j - l_i|
+//      exceeds 0.1 (B = X^T H X, R = I - X^T X, l_i = B_ii / (1 - R_ii)),
+//   2. diagonalises every cluster block of B exactly (parallel cyclic Jacobi
+//      in LDS, one workgroup per cluster) and rotates X by the block-diagonal
+//      result Q,
+//   3. takes one refinement step of Ogita and Aishima (Japan J. Indust. Appl.
+//      Math. 35, 2018) for all remaining pairs:
+//         E_ij = (B_ij + l_j R_ij)/(l_j - l_i) between clusters,
+//         E_ij = R_ij / 2 inside a cluster and on the diagonal,
+//         X <- X (I + E),
+// and stops when max |B_ij| (i != j) <= 1e-12 max |H| and max |R_ij| <= 1e-12.
+// Every step is an n x n FP64 GEMM or a single small kernel, instead of the
+// ~1,500 small kernels of the tridiagonal-reduction eigensolver.  A cluster
+// larger than kClusterStart at the start (kClusterMax later), too many strong
+// pairs, or no quadratic convergence within kRefineIters iterations falls back
+// to rocsolver_dsyevd.
 // ---------------------------------------------------------------------------
+constexpr int kClusterMax = 48;
+constexpr int kPairMax = 2048;
+constexpr int kRefineIters = 8;
+constexpr double kStrongPair = 0.1;
+constexpr int kClusterStart = 16;   // largest cluster accepted at the first iteration
+
 __device__ __forceinline__ void atomic_max_nonneg(double *addr, double v)
 {
     // non-negative doubles order like their bit patterns
     atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
-}
-
-// X = I + E0; st[0] = max off/gap ratio
-__global__ void k_rs_init(const double *__restrict__ H, int n, double *__restrict__ X, double *st)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n) return;
-    double x = 0.0;
-    if (i == j) {
-        x = 1.0;
-    } else {
-        const double gap = H[(long)j * n + j] - H[(long)i * n + i];
-        const double hij = H[(long)j * n + i];
-        const double ratio = fabs(gap) > 0.0 ? fabs(hij) / fabs(gap) : (hij == 0.0 ? 0.0 : 1e300);
-        atomic_max_nonneg(&st[0], ratio);
-        x = gap != 0.0 ? hij / gap : 0.0;
-    }
-    X[(long)j * n + i] = x;
 }
 
 __global__ void k_set_identity(double *A, int n)
@@ -130,7 +126,7 @@ __global__ void k_set_identity(double *A, int n)
     if (i < n) A[(long)j * n + i] = i == j ? 1.0 : 0.0;
 }
 
-// st[1] = |offdiag S|_F^2, st[2] = |R|_F^2, st[3] = |H|_F^2 (when hnorm)
+// st[1] = max |offdiag B|, st[2] = max |R|, st[3] = max |H| (when hnorm); R may be null
 __global__ void k_rs_norms(const double *__restrict__ S, const double *__restrict__ R,
                            const double *__restrict__ H, int n, int hnorm, double *st)
 {
@@ -139,46 +135,257 @@ __global__ void k_rs_norms(const double *__restrict__ S, const double *__restric
     const long nn = (long)n * n;
     for (long t = threadIdx.x; t < nn; t += blockDim.x) {
         const int j = (int)(t / n), i = (int)(t - (long)j * n);
-        if (i != j) a += S[t] * S[t];
-        b += R[t] * R[t];
-        if (hnorm) h += H[t] * H[t];
+        if (i != j) a = fmax(a, fabs(S[t]));
+        if (R) b = fmax(b, fabs(R[t]));
+        if (hnorm) h = fmax(h, fabs(H[t]));
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_down(a, off, 64);
-        b += __shfl_down(b, off, 64);
-        h += __shfl_down(h, off, 64);
+        a = fmax(a, __shfl_down(a, off, 64));
+        b = fmax(b, __shfl_down(b, off, 64));
+        h = fmax(h, __shfl_down(h, off, 64));
     }
     if (lane == 0) { red[0][w] = a; red[1][w] = b; red[2][w] = h; }
     __syncthreads();
     if (threadIdx.x == 0) {
         double s0 = 0, s1 = 0, s2 = 0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { s0 += red[0][k]; s1 += red[1][k]; s2 += red[2][k]; }
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+            s0 = fmax(s0, red[0][k]);
+            s1 = fmax(s1, red[1][k]);
+            s2 = fmax(s2, red[2][k]);
+        }
         st[1] = s0;
         st[2] = s1;
         if (hnorm) st[3] = s2;
     }
 }
 
-// F = I + E; lam_i = S_ii/(1 - R_ii); st[4] = number of pairs inside delta
+// Cluster search, one workgroup.  lam_i = B_ii/(1 - R_ii); strong pairs;
+// label propagation to connected components; compact member lists:
+// cl[0] = number of clusters (size >= 2), cl[1 .. ncl+1] = offsets,
+// cl[n+2 ..] = members.  clu[i] = cluster root of i (i itself if single).
+// st[5] = 1 on overflow (pairs or cluster size), st[6] = max cluster size,
+// st[7] = number of strong pairs.  R may be null (R = 0).
+__global__ void __launch_bounds__(1024)
+k_rs_clusters(const double *__restrict__ B, const double *__restrict__ R, int n,
+              double *__restrict__ lam, int *__restrict__ clu, int *__restrict__ cl, double *st)
+{
+    extern __shared__ unsigned char smem_raw[];
+    double *l = reinterpret_cast<double *>(smem_raw);
+    int *label = reinterpret_cast<int *>(l + n);
+    int *cnt = label + n;
+    int *pi = cnt + n;
+    int *pj = pi + kPairMax;
+    __shared__ int npairs, changed, ncl, fail, maxsz;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < n; i += nt) {
+        const double rii = R ? R[(long)i * n + i] : 0.0;
+        l[i] = B[(long)i * n + i] / (1.0 - rii);
+        lam[i] = l[i];
+        label[i] = i;
+        cnt[i] = 0;
+    }
+    if (tid == 0) { npairs = 0; fail = 0; maxsz = 0; }
+    __syncthreads();
+    const long nn = (long)n * n;
+    for (long t = tid; t < nn; t += nt) {
+        const int j = (int)(t / n), i = (int)(t - (long)j * n);
+        if (i >= j) continue;
+        const double num = B[t] + l[j] * (R ? R[t] : 0.0);
+        const double gap = l[j] - l[i];
+        if (fabs(num) > kStrongPair * fabs(gap)) {
+            const int k = atomicAdd(&npairs, 1);
+            if (k < kPairMax) { pi[k] = i; pj[k] = j; }
+        }
+    }
+    __syncthreads();
+    const int np = npairs < kPairMax ? npairs : kPairMax;
+    if (tid == 0 && npairs > kPairMax) fail = 1;
+    do {
+        __syncthreads();
+        if (tid == 0) changed = 0;
+        __syncthreads();
+        for (int k = tid; k < np; k += nt) {
+            const int a = label[pi[k]], b = label[pj[k]];
+            if (a != b) {
+                const int m = a < b ? a : b;
+                atomicMin(&label[pi[k]], m);
+                atomicMin(&label[pj[k]], m);
+                atomicMin(&label[a], m);
+                atomicMin(&label[b], m);
+                changed = 1;
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += nt) {   // pointer jumping
+            const int a = label[i], b = label[a];
+            if (b < a) { label[i] = b; changed = 1; }
+        }
+        __syncthreads();
+    } while (changed);
+    for (int i = tid; i < n; i += nt) atomicAdd(&cnt[label[i]], 1);
+    __syncthreads();
+    if (tid == 0) {
+        // roots of clusters of size >= 2 in index order; cnt[root] becomes its offset
+        int off = 0, c = 0;
+        for (int i = 0; i < n; ++i) {
+            const int sz = cnt[i];
+            if (label[i] == i && sz >= 2) {
+                if (sz > maxsz) maxsz = sz;
+                if (sz > kClusterMax) fail = 1;
+                cl[1 + c] = off;
+                cnt[i] = off;
+                off += sz;
+                ++c;
+            } else {
+                cnt[i] = -1;
+            }
+        }
+        cl[1 + c] = off;
+        cl[0] = c;
+        ncl = c;
+        st[5] = fail ? 1.0 : 0.0;
+        st[6] = maxsz;
+        st[7] = npairs;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += nt) {
+        const int r = label[i];
+        clu[i] = r;
+        if (cnt[r] >= 0) {
+            const int pos = atomicAdd(&cnt[r], 1);
+            cl[n + 2 + pos] = i;
+        }
+    }
+}
+
+// Members of one cluster in ascending index order (atomics scatter them).
+__device__ void sort_members(int *idx, int m)
+{
+    if (threadIdx.x == 0)
+        for (int a = 1; a < m; ++a) {
+            const int v = idx[a];
+            int b = a - 1;
+            while (b >= 0 && idx[b] > v) { idx[b + 1] = idx[b]; --b; }
+            idx[b + 1] = v;
+        }
+    __syncthreads();
+}
+
+// Q(cluster, cluster) = eigenvectors of B(cluster, cluster) by parallel cyclic
+// Jacobi (round-robin pairing, m/2 disjoint rotations per round).  Q must be
+// the identity outside the clusters.  st[4] += sweeps that did not converge.
+__global__ void __launch_bounds__(256)
+k_rs_jacobi(const double *__restrict__ B, int n, const int *__restrict__ cl, double *__restrict__ Q,
+            double *st)
+{
+    __shared__ double A[kClusterMax][kClusterMax + 1];
+    __shared__ double V[kClusterMax][kClusterMax + 1];
+    __shared__ double cs[kClusterMax / 2], sn[kClusterMax / 2];
+    __shared__ int pp[kClusterMax / 2], qq[kClusterMax / 2];
+    __shared__ int idx[kClusterMax];
+    __shared__ double offn, diagn;
+    const int ncl = cl[0];
+    if ((int)blockIdx.x >= ncl) return;
+    const int b0 = cl[1 + blockIdx.x], m = cl[2 + blockIdx.x] - b0;
+    if (m > kClusterMax) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int a = tid; a < m; a += nt) idx[a] = cl[n + 2 + b0 + a];
+    __syncthreads();
+    sort_members(idx, m);
+    for (int t = tid; t < m * m; t += nt) {
+        const int a = t % m, b = t / m;
+        A[a][b] = B[(long)idx[b] * n + idx[a]];
+        V[a][b] = a == b ? 1.0 : 0.0;
+    }
+    const int me = m + (m & 1);   // even size; index m (odd m) is a dummy
+    const int np = me / 2;
+    int sweep = 0;
+    for (; sweep < 40; ++sweep) {
+        __syncthreads();
+        if (tid == 0) { offn = 0.0; diagn = 0.0; }
+        __syncthreads();
+        double o = 0.0, d = 0.0;
+        for (int t = tid; t < m * m; t += nt) {
+            const int a = t % m, b = t / m;
+            if (a != b) o += A[a][b] * A[a][b];
+            else d += A[a][b] * A[a][b];
+        }
+        atomicAdd(&offn, o);
+        atomicAdd(&diagn, d);
+        __syncthreads();
+        if (offn <= 1e-30 * diagn || offn == 0.0) break;
+        for (int rnd = 0; rnd < me - 1; ++rnd) {
+            if (tid < np) {
+                // round-robin: position 0 fixed, the others rotate
+                const int k = tid;
+                int p = k == 0 ? 0 : 1 + (k - 1 + rnd) % (me - 1);
+                int q = 1 + (me - 2 - k + rnd) % (me - 1);
+                if (p > q) { const int t2 = p; p = q; q = t2; }
+                double c = 1.0, s = 0.0;
+                if (q < m && A[p][q] != 0.0) {
+                    const double apq = A[p][q];
+                    const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+                    const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    c = 1.0 / sqrt(t * t + 1.0);
+                    s = t * c;
+                }
+                pp[k] = p; qq[k] = q; cs[k] = c; sn[k] = s;
+            }
+            __syncthreads();
+            // columns: A <- A J, V <- V J
+            for (int t = tid; t < np * m; t += nt) {
+                const int k = t / m, r = t - k * m;
+                const int p = pp[k], q = qq[k];
+                if (q >= m) continue;
+                const double c = cs[k], s = sn[k];
+                const double ap = A[r][p], aq = A[r][q];
+                A[r][p] = c * ap - s * aq;
+                A[r][q] = s * ap + c * aq;
+                const double vp = V[r][p], vq = V[r][q];
+                V[r][p] = c * vp - s * vq;
+                V[r][q] = s * vp + c * vq;
+            }
+            __syncthreads();
+            // rows: A <- J^T A
+            for (int t = tid; t < np * m; t += nt) {
+                const int k = t / m, r = t - k * m;
+                const int p = pp[k], q = qq[k];
+                if (q >= m) continue;
+                const double c = cs[k], s = sn[k];
+                const double ap = A[p][r], aq = A[q][r];
+                A[p][r] = c * ap - s * aq;
+                A[q][r] = s * ap + c * aq;
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0 && sweep >= 40) atomicAdd(&st[4], 1.0);
+    for (int t = tid; t < m * m; t += nt) {
+        const int a = t % m, b = t / m;
+        Q[(long)idx[b] * n + idx[a]] = V[a][b];
+    }
+}
+
+// F = I + E of the refinement step; lam_i = B_ii/(1 - R_ii)
 __global__ void k_rs_update(const double *__restrict__ S, const double *__restrict__ R, int n,
-                            double *__restrict__ F, double *__restrict__ lam, double *st)
+                            const int *__restrict__ clu, double *__restrict__ F,
+                            double *__restrict__ lam)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (i >= n) return;
     const double li = S[(long)i * n + i] / (1.0 - R[(long)i * n + i]);
     const double lj = S[(long)j * n + j] / (1.0 - R[(long)j * n + j]);
     if (i == j) lam[i] = li;
-    const double delta = 2.0 * (sqrt(st[1]) + sqrt(st[3]) * sqrt(st[2]));
-    double e;
     const double rij = R[(long)j * n + i];
-    if (i == j) {
+    double e;
+    if (i == j || clu[i] == clu[j]) {
         e = 0.5 * rij;
-    } else if (fabs(li - lj) > delta) {
-        e = (S[(long)j * n + i] + lj * rij) / (lj - li);
     } else {
-        e = 0.5 * rij;
-        atomicAdd(&st[4], 1.0);
+        e = (S[(long)j * n + i] + lj * rij) / (lj - li);
+        // a pair that became strong after the cluster rotation waits for the
+        // next cluster search instead of taking a large first-order step
+        if (!(fabs(e) <= 0.3)) e = 0.5 * rij;
     }
     F[(long)j * n + i] = (i == j ? 1.0 : 0.0) + e;
 }
@@ -212,43 +419,78 @@ static int read_st(double *host, const double *dev, int n)
     return kOK;
 }
 
+// B = X^T H X, R = I - X^T X (T is scratch)
+static int rs_project(const double *H, const double *X, int n, double *T, double *B, double *R)
+{
+    Context &c = ctx();
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    const dim3 g2((n + 127) / 128, n), b2(128);
+    RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, H,
+                          n, X, n, &zero, T, n));
+    RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one,
+                          X, n, T, n, &zero, B, n));
+    hipLaunchKernelGGL(k_set_identity, g2, b2, 0, c.stream, R, n);
+    RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, n, n, n,
+                          &mone, X, n, X, n, &one, R, n));
+    return kOK;
+}
+
 // On success V (n x n) and D (n) hold the eigenpairs of H, *ok = 1.
 static int refine_eigh(const double *H, int n, double *V, double *D, int *ok)
 {
     Context &c = ctx();
     *ok = 0;
     const size_t nn = (size_t)n * n;
-    RS_TRY(c.evwork.ensure(sizeof(double) * (6 * nn + n + 8)));
-    double *X = c.evwork.as<double>(), *B = X + nn, *S = B + nn, *R = S + nn, *F = R + nn,
+    const size_t lds = (size_t)n * sizeof(double) + 2 * (size_t)n * sizeof(int) +
+                       2 * (size_t)kPairMax * sizeof(int);
+    if (lds > 64 * 1024) return kOK;   // cluster search does not fit one workgroup
+    RS_TRY(c.evwork.ensure(sizeof(double) * (6 * nn + n + 8) + sizeof(int) * (3 * n + 4)));
+    double *X = c.evwork.as<double>(), *T = X + nn, *B = T + nn, *R = B + nn, *F = R + nn,
            *Xn = F + nn, *lam = Xn + nn, *st = lam + n;
+    int *clu = reinterpret_cast<int *>(st + 8), *cl = clu + n;
     const dim3 g2((n + 127) / 128, n), b2(128);
+    const double one = 1.0, zero = 0.0;
     RS_HIP(hipMemsetAsync(st, 0, sizeof(double) * 8, c.stream));
-    hipLaunchKernelGGL(k_rs_init, g2, b2, 0, c.stream, H, n, X, st);
+    hipLaunchKernelGGL(k_rs_norms, dim3(1), dim3(1024), 0, c.stream, H, nullptr, H, n, 1, st);
+    hipLaunchKernelGGL(k_set_identity, g2, b2, 0, c.stream, X, n);
+    const double *Bc = H, *Rc = nullptr;   // B and R of the current X (X = I)
     double hs[8];
-    RS_TRY(read_st(hs, st, 8));
-    if (!(hs[0] < 0.3)) return kOK;
-    const double one = 1.0, zero = 0.0, mone = -1.0;
     bool converged = false;
-    for (int it = 0; it < 10; ++it) {
-        RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
-                              H, n, X, n, &zero, B, n));
-        RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, n, n, n,
-                              &one, X, n, B, n, &zero, S, n));
-        hipLaunchKernelGGL(k_set_identity, g2, b2, 0, c.stream, R, n);
-        RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, n, n, n,
-                              &mone, X, n, X, n, &one, R, n));
-        RS_HIP(hipMemsetAsync(st + 4, 0, sizeof(double), c.stream));
-        hipLaunchKernelGGL(k_rs_norms, dim3(1), dim3(1024), 0, c.stream, S, R, H, n, it == 0 ? 1 : 0, st);
-        hipLaunchKernelGGL(k_rs_update, g2, b2, 0, c.stream, S, R, n, F, lam, st);
-        RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
-                              X, n, F, n, &zero, Xn, n));
+    for (int it = 0; it < kRefineIters; ++it) {
+        // 1-2: exact rotation inside every cluster
+        hipLaunchKernelGGL(k_rs_clusters, dim3(1), dim3(1024), lds, c.stream, Bc, Rc, n, lam, clu,
+                           cl, st);
+        if (it == 0) {
+            // large clusters at the start mean the Ritz vectors are still moving:
+            // the tridiagonal eigensolver is cheaper than many refinement steps
+            RS_TRY(read_st(hs, st, 8));
+            if (hs[5] != 0.0 || hs[6] > kClusterStart) return kOK;
+        }
+        hipLaunchKernelGGL(k_set_identity, g2, b2, 0, c.stream, F, n);
+        hipLaunchKernelGGL(k_rs_jacobi, dim3(n / 2), dim3(256), 0, c.stream, Bc, n, cl, F, st);
+        RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_none, n, n, n,
+                              &one, X, n, F, n, &zero, Xn, n));
         std::swap(X, Xn);
+        RS_TRY(rs_project(H, X, n, T, B, R));
+        // 3: refinement step between clusters
+        hipLaunchKernelGGL(k_rs_update, g2, b2, 0, c.stream, B, R, n, clu, F, lam);
+        RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_none, n, n, n,
+                              &one, X, n, F, n, &zero, Xn, n));
+        std::swap(X, Xn);
+        RS_TRY(rs_project(H, X, n, T, B, R));
+        Bc = B;
+        Rc = R;
+        hipLaunchKernelGGL(k_rs_norms, dim3(1), dim3(1024), 0, c.stream, B, R, H, n, 0, st);
+        RS_HIP(hipGetLastError());
         RS_TRY(read_st(hs, st, 8));
-        const double hn = sqrt(hs[3]);
-        if (hs[4] > 0.0 && it >= 3) break;                       // unresolved cluster
-        if (sqrt(hs[1]) <= 1e-12 * hn && sqrt(hs[2]) <= 1e-12) { converged = hs[4] == 0.0; break; }
+        if (hs[5] != 0.0 || hs[4] != 0.0) return kOK;   // cluster too large / Jacobi stalled
+        // rounding level: the next Gram-Schmidt restores orthogonality below 1e-12 anyway
+        if (hs[1] <= 1e-12 * hs[3] && hs[2] <= 1e-12) { converged = true; break; }
+        if (it >= 2 && hs[1] > 1e-5 * hs[3]) return kOK;   // not converging quadratically
     }
     if (!converged) return kOK;
+    // eigenvalues: diagonal of the final B
+    hipLaunchKernelGGL(k_rs_update, g2, b2, 0, c.stream, B, R, n, clu, F, lam);
     hipLaunchKernelGGL(k_rs_sort, dim3(n), dim3(256), 0, c.stream, X, lam, n, V, D);
     RS_HIP(hipGetLastError());
     *ok = 1;
@@ -339,7 +581,10 @@ int subspace_diag_dev(T *X, int nb, double *esp_dev)
     if (c.eig_mode == 1) {
         RS_TRY(c.mat3.ensure(sizeof(double) * (size_t)nb * nb));
         RS_TRY(refine_eigh(H, nb, c.mat3.as<double>(), D, &refined));
-        if (refined) Vec = c.mat3.as<double>();
+        if (refined) {
+            Vec = c.mat3.as<double>();
+            hipLaunchKernelGGL(k_fix_signs, dim3(nb), dim3(128), 0, c.stream, Vec, nb);
+        }
         c.eig_refined += refined;
     }
     if (!refined) {

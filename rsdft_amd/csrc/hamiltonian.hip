@@ -2,20 +2,23 @@
 // Kleinman-Bylander nonlocal projectors, and the 7-point preconditioner
 // operator.
 //
-// Stencil design (one band per workgroup column):
+// Stencil design (one band per workgroup column, kernels in stencil_kernels.h):
 //   * a workgroup owns `ty` full x-rows of the (x,y) plane and marches `tz`
-//     planes in z; every thread owns `PX` consecutive x points of one row;
-//   * the 2*Md+1 z-neighbours of a thread's points live in a register queue
-//     (one new plane loaded per step), so z costs no LDS traffic;
-//   * the current plane (own rows + Md-deep y halo, full x so the x wrap is
-//     a modular index into LDS) is staged in LDS for the x and y terms;
+//     planes in z; every thread owns 2 x points of one row (FP32: a
+//     consecutive pair moved as one 8-byte access; FP64: two points N1/2
+//     apart, so lanes touch consecutive words);
+//   * the 2*Md+1 z-neighbours of a thread's points live in a register queue,
+//     loaded PF planes ahead, so z costs no LDS traffic;
+//   * the current plane (own rows + Md-deep y halo, x wrapped into pads) is
+//     staged in one of two LDS buffers for the x and y terms: one barrier per
+//     plane;
 //   * periodic wrap in y/z is modular indexing of the global loads: the
 //     reference's bcset_3 halo exchange (bcset_3_module.f90:15-151,
 //     bc_module.f90:722-736) has no counterpart;
 //   * the epilogue adds Vloc*psi (op_localpot);
 //   * the nonlocal term is added afterwards by k_nl_apply, which visits only
-//     the grid points inside some projector sphere, for many bands per
-//     entry load.
+//     the grid points inside the projector spheres, a group of projectors
+//     sharing a sphere at a time, for many bands per table load.
 #include "common.h"
 
 #include <algorithm>
@@ -28,115 +31,152 @@ namespace rsdft {
 
 namespace rsdft {
 
+template <class T, int P>
+struct SlotVec {
+    T v[P];
+};
+
 // ---------------------------------------------------------------------------
-// Nonlocal projection: coefC[lma][b] = iuV(lma)*dV*sum_j uV(j,lma) psi_b(JJP(j,lma))
-// (ps_nloc2_op_module.f90:161-175), FP64 accumulation, NBB bands per block.
+// Nonlocal projection (ps_nloc2_op_module.f90:161-175):
+//   coefC[lma][b] = iuV(lma)*dV * sum_j uV(j,lma) psi_b(JJP(j,lma))
+// One wave per (projector group, NBB bands): each grid value of the group's
+// union point list is loaded once and multiplied into all P channels, FP64
+// accumulation.  The 64 lanes' partial sums are added by a fixed butterfly,
+// so the result does not depend on scheduling.
 // ---------------------------------------------------------------------------
-template <int NBB>
-__device__ __forceinline__ void block_reduce_store(double (&acc)[NBB], double *red) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+// Sum of v[0..K) over the 64 lanes of a wave, K a power of two <= 64, by
+// recursive halving: at distance d every lane keeps the half of its values
+// that its side of the d-split owns and adds the partner's copy of that half,
+// so after log2(K) steps lane l holds value l / (64/K) summed over 64/(64/K)
+// lanes; the remaining distances complete the sum.  Fixed order, no LDS.
+template <int D, int N, int K>
+__device__ __forceinline__ void reduce_scatter_step(double (&v)[K], int lane)
+{
+    if constexpr (N > 1) {
+        constexpr int H = N / 2;
+        const bool up = (lane & D) != 0;
 #pragma unroll
-    for (int k = 0; k < NBB; ++k) {
-        double v = acc[k];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0) red[wave * NBB + k] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < NBB) {
-        double s = 0.0;
-        for (int w = 0; w < nw; ++w) s += red[w * NBB + threadIdx.x];
-        acc[0] = s;   // thread k holds the total of band k in acc[0]
+        for (int i = 0; i < H; ++i) {
+            const double send = up ? v[i] : v[i + H];
+            const double keep = up ? v[i + H] : v[i];
+            v[i] = keep + __shfl_xor(send, D, 64);
+        }
+        reduce_scatter_step<D / 2, H, K>(v, lane);
     }
 }
 
-template <class T, int NBB>
-__global__ void __launch_bounds__(256)
-k_project(const T *__restrict__ in, long ML, int nb, int nz, const int *__restrict__ rptr,
-          const int *__restrict__ ridx, const T *__restrict__ rval,
-          const double *__restrict__ scale, T *__restrict__ coefC,
-          const int *__restrict__ band_active)
+template <int K>
+__device__ __forceinline__ double wave_reduce_scatter(double (&v)[K])
 {
-    __shared__ double red[16 * NBB];
-    const int nbblk = (nb + NBB - 1) / NBB;
-    const long total = (long)nz * nbblk;
-    const long w = xcd_work_index(blockIdx.x, total);
-    const int lma = (int)(w % nz);
-    const int b0 = (int)(w / nz) * NBB;
-    double acc[NBB];
+    const int lane = threadIdx.x & 63;
+    reduce_scatter_step<32, K, K>(v, lane);
+    double r = v[0];
 #pragma unroll
-    for (int k = 0; k < NBB; ++k) acc[k] = 0.0;
+    for (int d = 32 / K; d > 0; d >>= 1) r += __shfl_xor(r, d, 64);
+    return r;
+}
+
+template <class T, int P, int NBB>
+__global__ void __launch_bounds__(64)
+k_nl_project(const T *__restrict__ in, long ML, int nb, int ng, const int *__restrict__ gptr,
+             const int *__restrict__ gidx, const T *__restrict__ gval, const int *__restrict__ glma,
+             const double *__restrict__ scale, T *__restrict__ coefC,
+             const int *__restrict__ band_active)
+{
+    constexpr int K = P * NBB;
+    static_assert(K <= 64 && (K & (K - 1)) == 0, "P*NBB must be a power of two <= 64");
+    const int nbblk = (nb + NBB - 1) / NBB;
+    const long total = (long)ng * nbblk;
+    const long w = xcd_work_index(blockIdx.x, total);
+    if (w >= total) return;
+    const int g = (int)(w % ng);
+    const int b0 = (int)(w / ng) * NBB;
+    const int lane = threadIdx.x;
     bool act[NBB];
 #pragma unroll
-    for (int k = 0; k < NBB; ++k)
-        act[k] = (b0 + k < nb) && (!band_active || band_active[b0 + k]);
-    const int j1 = rptr[lma + 1];
-    for (int j = rptr[lma] + threadIdx.x; j < j1; j += blockDim.x) {
-        const int i = ridx[j];
-        const double v = (double)rval[j];
+    for (int k = 0; k < NBB; ++k) act[k] = (b0 + k < nb) && (!band_active || band_active[b0 + k]);
+    double acc[K];
 #pragma unroll
-        for (int k = 0; k < NBB; ++k)
-            if (act[k]) acc[k] += v * (double)in[(long)(b0 + k) * ML + i];
+    for (int t = 0; t < K; ++t) acc[t] = 0.0;
+    const int j1 = gptr[g + 1];
+    for (int j = gptr[g] + lane; j < j1; j += 64) {
+        const long i = gidx[j];
+        SlotVec<T, P> u;
+#pragma unroll
+        for (int s = 0; s < P; ++s) u.v[s] = gval[(long)j * P + s];
+#pragma unroll
+        for (int k = 0; k < NBB; ++k) {
+            if (!act[k]) continue;
+            const double x = (double)in[(long)(b0 + k) * ML + i];
+#pragma unroll
+            for (int s = 0; s < P; ++s) acc[s * NBB + k] += (double)u.v[s] * x;
+        }
     }
-    block_reduce_store<NBB>(acc, red);
-    if (threadIdx.x < NBB) {
-        const int b = b0 + threadIdx.x;
-        if (b < nb && (!band_active || band_active[b]))
-            coefC[(long)lma * nb + b] = (T)(scale[lma] * acc[0]);
+    // lane l ends with the total of value t = l / (64/K)
+    const double sum = wave_reduce_scatter<K>(acc);
+    constexpr int R = 64 / K;
+    if (lane % R == 0) {
+        const int t = lane / R;
+        const int s = t / NBB, k = t - s * NBB;
+        const int lma = glma[g * P + s];
+        const int b = b0 + k;
+        if (lma >= 0 && b < nb && (!band_active || band_active[b]))
+            coefC[(long)lma * nb + b] = (T)(scale[lma] * sum);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Nonlocal apply (ps_nloc2_op_module.f90:255-263):
 //   htpsi(i,b) += sum_{lma covering i} uV(i,lma) * coefC[lma][b]
-// over the grid points that lie in some sphere ("covered" points).  A
-// workgroup stages the point-major entries (lma, uV) of its 256 points in LDS
-// once and sweeps its band range NBB bands at a time, so the tables are read
-// once per workgroup instead of once per band.  Entries of a point are summed
-// in lma order, as the reference accumulates them.
+// per group of one colour (disjoint point lists, so no two workgroups of a
+// launch touch the same output element): every thread owns grid points of
+// the group and adds the group's P channels for NBB bands.
 // ---------------------------------------------------------------------------
-template <class T, int NBB>
+template <class T, int P, int NBB>
 __global__ void __launch_bounds__(256)
-k_nl_apply(T *__restrict__ out, long ML, int nb, int bands_per_group,
-           const int *__restrict__ cov_pts, const int *__restrict__ cov_ptr, int ncov,
-           const int *__restrict__ clma, const T *__restrict__ cval,
-           const T *__restrict__ coefC, const int *__restrict__ band_active)
+k_nl_apply(T *__restrict__ out, long ML, int nb, const int *__restrict__ order, int ngc,
+           const int *__restrict__ gptr, const int *__restrict__ gidx, const T *__restrict__ gval,
+           const int *__restrict__ glma, const T *__restrict__ coefC,
+           const int *__restrict__ band_active)
 {
-    extern __shared__ unsigned char smem_raw[];
-    const int p0 = blockIdx.x * 256;
-    const int npts = min(256, ncov - p0);
-    const int e0 = cov_ptr[p0], e1 = cov_ptr[p0 + npts];
-    const int ne = e1 - e0;
-    int *Llma = reinterpret_cast<int *>(smem_raw);
-    T *Lval = reinterpret_cast<T *>(smem_raw + (((size_t)ne * sizeof(int) + 15) & ~(size_t)15));
-    for (int e = threadIdx.x; e < ne; e += 256) {
-        Llma[e] = clma[e0 + e];
-        Lval[e] = cval[e0 + e];
+    const int nbblk = (nb + NBB - 1) / NBB;
+    const long total = (long)ngc * nbblk;
+    const long w = xcd_work_index(blockIdx.x, total);
+    if (w >= total) return;
+    const int g = order[w % ngc];
+    const int b0 = (int)(w / ngc) * NBB;
+    bool act[NBB];
+#pragma unroll
+    for (int k = 0; k < NBB; ++k) act[k] = (b0 + k < nb) && (!band_active || band_active[b0 + k]);
+    // the P x NBB coefficients are uniform; reading them back from LDS keeps
+    // them in VGPRs (as scalars they would not fit the SGPR file)
+    __shared__ T Lcf[P * NBB];
+    for (int t = threadIdx.x; t < P * NBB; t += blockDim.x) {
+        const int s = t / NBB, k = t - s * NBB;
+        const int lma = glma[g * P + s];
+        Lcf[t] = (lma >= 0 && b0 + k < nb && (!band_active || band_active[b0 + k]))
+                     ? coefC[(long)lma * nb + b0 + k] : T(0);
     }
     __syncthreads();
-    const int t = threadIdx.x;
-    if (t >= npts) return;
-    const long pt = cov_pts[p0 + t];
-    const int le0 = cov_ptr[p0 + t] - e0, le1 = cov_ptr[p0 + t + 1] - e0;
-    const int bg0 = blockIdx.y * bands_per_group;
-    const int bg1 = min(nb, bg0 + bands_per_group);
-    for (int b0 = bg0; b0 < bg1; b0 += NBB) {
-        T acc[NBB];
+    T cf[P][NBB];
 #pragma unroll
-        for (int k = 0; k < NBB; ++k) acc[k] = T(0);
-        for (int e = le0; e < le1; ++e) {
-            const T v = Lval[e];
-            const T *cr = coefC + (long)Llma[e] * nb + b0;
+    for (int s = 0; s < P; ++s)
 #pragma unroll
-            for (int k = 0; k < NBB; ++k)
-                if (b0 + k < bg1) acc[k] = acc[k] + v * cr[k];
-        }
+        for (int k = 0; k < NBB; ++k) cf[s][k] = Lcf[s * NBB + k];
+    const int j1 = gptr[g + 1];
+    for (int j = gptr[g] + threadIdx.x; j < j1; j += blockDim.x) {
+        const long i = gidx[j];
+        SlotVec<T, P> u;
+#pragma unroll
+        for (int s = 0; s < P; ++s) u.v[s] = gval[(long)j * P + s];
 #pragma unroll
         for (int k = 0; k < NBB; ++k) {
-            const int b = b0 + k;
-            if (b < bg1 && (!band_active || band_active[b])) {
-                T *o = out + (long)b * ML + pt;
-                *o = *o + acc[k];
-            }
+            if (!act[k]) continue;
+            T a = T(0);
+#pragma unroll
+            for (int s = 0; s < P; ++s) a = a + u.v[s] * cf[s][k];
+            T *o = out + (long)(b0 + k) * ML + i;
+            *o = *o + a;
         }
     }
 }
@@ -307,6 +347,63 @@ StencilGeom choose_geom(int N1, int N2, int N3, int md, int px, int elem_bytes)
     return g;
 }
 
+StencilGeom choose_hpsi_geom(int N1, int N2, int N3, int md, int px, int pad, int elem_bytes,
+                             int max_threads)
+{
+    StencilGeom g;
+    g.px = px;
+    g.threads = 0;
+    if (N1 % px) return g;
+    const int tpr = N1 / px;
+    double best = -1.0;
+    for (int ty = md; ty <= N2; ++ty) {   // ty >= md keeps halo items <= 2 per thread
+        if (N2 % ty) continue;
+        const int thr = tpr * ty;
+        if (thr > max_threads) break;
+        const size_t lds = 2 * (size_t)(ty + 2 * md) * (N1 + 2 * pad) * elem_bytes;
+        if (lds > 64 * 1024) continue;
+        const int waves = (thr + 63) / 64;
+        const double eff = (double)thr / (64.0 * waves);
+        // one halo item per thread first, then full waves, then taller tiles
+        const double score = (ty >= 2 * md ? 8.0 : 0.0) + eff * 4.0 + (double)ty / (ty + 2.0 * md) +
+                             (thr >= 128 ? 1.0 : 0.0);
+        if (score > best) {
+            best = score;
+            g.ty = ty;
+            g.threads = thr;
+            g.lds = lds;
+        }
+    }
+    g.hm = g.ty >= 2 * md ? 1 : 2;
+    // z-march: the whole column when it is short, else the longest divisor <= 96
+    int tz = N3;
+    if (N3 > 96)
+        for (int d = 96; d >= 1; --d)
+            if (N3 % d == 0) { tz = d; break; }
+    g.tz = tz;
+    g.tiles = (N2 / g.ty) * (N3 / g.tz);
+    return g;
+}
+
+template <class T, int MD, int HM, int FL>
+static void launch_hpsi_k(const T *in, T *out, int nb, const int *band_active, const T *vloc,
+                          const StencilCoef<T, MD> &cf)
+{
+    Context &c = ctx();
+    const GridDesc &g = c.g;
+    const StencilGeom &G = sizeof(T) == 8 ? c.geom_h : c.geom_h32;
+    const long nblk = (long)G.tiles * nb;
+    const dim3 grid((unsigned)((nblk + 7) / 8 * 8));
+    if constexpr (sizeof(T) == 8)
+        hipLaunchKernelGGL((k_hpsi_s<T, MD, kHpsiPx64, HM, FL, 1>), grid, dim3(G.threads), G.lds,
+                           c.stream, in, out, g.ML, g.N1, g.N2, g.N3, G.ty, G.tz, G.tiles, nb, cf,
+                           vloc, band_active);
+    else
+        hipLaunchKernelGGL((k_hpsi2<T, MD, kHpsiPx32, HM, FL, kHpsiThreads32>), grid,
+                           dim3(G.threads), G.lds, c.stream, in, out, g.ML, g.N1, g.N2, g.N3, G.ty,
+                           G.tz, G.tiles, nb, cf, vloc, band_active);
+}
+
 template <class T, int MD>
 static int launch_hpsi_md(const T *in, T *out, int nb, int flags, const int *band_active,
                           const T *vloc)
@@ -314,7 +411,6 @@ static int launch_hpsi_md(const T *in, T *out, int nb, int flags, const int *ban
     Context &c = ctx();
     const GridDesc &g = c.g;
     const StencilGeom &G = sizeof(T) == 8 ? c.geom_h : c.geom_h32;
-    constexpr int PX = sizeof(T) == 8 ? 2 : 4;
     StencilCoef<T, MD> cf;
     cf.c0 = (T)g.coef_lap0;
     for (int m = 0; m < MD; ++m) {
@@ -322,14 +418,33 @@ static int launch_hpsi_md(const T *in, T *out, int nb, int flags, const int *ban
         cf.cy[m] = (T)g.coef_lap[1][m];
         cf.cz[m] = (T)g.coef_lap[2][m];
     }
-    const int tpr = g.N1 / PX;
-    const size_t lds = (size_t)(G.ty + 2 * MD) * (g.N1 + 2 * MD) * sizeof(T);
-    const long nblk = (long)G.tiles * nb;
-    hipLaunchKernelGGL((k_hpsi<T, MD, PX, 2>), dim3((nblk + 7) / 8 * 8), dim3(G.threads), lds,
-                       c.stream, in, out, g.ML, g.N1, g.N2, g.N3, G.ty, G.tz, tpr, G.tiles, nb, cf,
-                       vloc, flags, band_active);
+    const bool h1 = G.hm == 1;
+    if (flags == (kKin | kLoc)) {
+        if (h1) launch_hpsi_k<T, MD, 1, kKin | kLoc>(in, out, nb, band_active, vloc, cf);
+        else launch_hpsi_k<T, MD, 2, kKin | kLoc>(in, out, nb, band_active, vloc, cf);
+    } else if (flags == kKin) {
+        if (h1) launch_hpsi_k<T, MD, 1, kKin>(in, out, nb, band_active, vloc, cf);
+        else launch_hpsi_k<T, MD, 2, kKin>(in, out, nb, band_active, vloc, cf);
+    } else {
+        return set_error(kErrArg, "stencil flags %d not supported", flags);
+    }
     RS_HIP(hipGetLastError());
     return kOK;
+}
+
+// out[b][i] += V[i] * in[b][i]  (op_localpot alone, localpot_module.f90:27-45)
+template <class T>
+__global__ void __launch_bounds__(256)
+k_localpot_acc(const T *__restrict__ in, T *__restrict__ out, const T *__restrict__ vloc, long ML,
+               int nb, const int *__restrict__ band_active)
+{
+    const long n = ML * nb;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        const long b = i / ML;
+        if (band_active && !band_active[b]) continue;
+        out[i] = out[i] + vloc[i - b * ML] * in[i];
+    }
 }
 
 template <class T>
@@ -338,6 +453,14 @@ static int launch_stencil(const T *in, T *out, int nb, int flags, const int *ban
     Context &c = ctx();
     const T *vloc = sizeof(T) == 8 ? (const T *)c.vloc64.p : (const T *)c.vloc32.p;
     if ((flags & kLoc) && !vloc) return set_error(kErrState, "local potential not set");
+    if (flags == (kLoc | kAcc)) {
+        const long n = c.g.ML * nb;
+        const int blocks = (int)std::min<long>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL((k_localpot_acc<T>), dim3(blocks), dim3(256), 0, c.stream, in, out,
+                           vloc, c.g.ML, nb, band_active);
+        RS_HIP(hipGetLastError());
+        return kOK;
+    }
     switch (c.g.Md) {
     case 1: return launch_hpsi_md<T, 1>(in, out, nb, flags, band_active, vloc);
     case 2: return launch_hpsi_md<T, 2>(in, out, nb, flags, band_active, vloc);
@@ -349,17 +472,52 @@ static int launch_stencil(const T *in, T *out, int nb, int flags, const int *ban
     }
 }
 
+template <class T, int P>
+static void launch_nl_project(const T *in, int nb, T *coef, const int *band_active)
+{
+    Context &c = ctx();
+    NonlocalTables &nl = c.nl;
+    constexpr int NBB = P >= 16 ? 2 : 32 / P > 16 ? 16 : 32 / P;
+    const long nblk = (long)nl.ng * ((nb + NBB - 1) / NBB);
+    const T *val = sizeof(T) == 8 ? (const T *)nl.g_val64.p : (const T *)nl.g_val32.p;
+    hipLaunchKernelGGL((k_nl_project<T, P, NBB>), dim3((unsigned)((nblk + 7) / 8 * 8)), dim3(64), 0,
+                       c.stream, in, c.g.ML, nb, nl.ng, nl.g_ptr.as<int>(), nl.g_idx.as<int>(), val,
+                       nl.g_lma.as<int>(), nl.scale64.as<double>(), coef, band_active);
+}
+
+template <class T, int P>
+static void launch_nl_apply(T *out, int nb, const T *coef, const int *band_active)
+{
+    Context &c = ctx();
+    NonlocalTables &nl = c.nl;
+    // P*NBB coefficients per thread: 64 FP32 / 32 FP64 registers
+    constexpr int NBB = sizeof(T) == 8 ? (P >= 8 ? 2 : 32 / P / 2 * 2 > 8 ? 8 : 32 / P)
+                                       : (P >= 16 ? 4 : 64 / P > 16 ? 16 : 64 / P);
+    const T *val = sizeof(T) == 8 ? (const T *)nl.g_val64.p : (const T *)nl.g_val32.p;
+    for (size_t col = 0; col + 1 < nl.color_ptr.size(); ++col) {
+        const int g0 = nl.color_ptr[col], ngc = nl.color_ptr[col + 1] - g0;
+        if (ngc <= 0) continue;
+        const long nblk = (long)ngc * ((nb + NBB - 1) / NBB);
+        hipLaunchKernelGGL((k_nl_apply<T, P, NBB>), dim3((unsigned)((nblk + 7) / 8 * 8)), dim3(256),
+                           0, c.stream, out, c.g.ML, nb, nl.g_order.as<int>() + g0, ngc,
+                           nl.g_ptr.as<int>(), nl.g_idx.as<int>(), val, nl.g_lma.as<int>(), coef,
+                           band_active);
+    }
+}
+
 template <class T>
 int nl_project(const T *in, int nb, T *coef, const int *band_active)
 {
     Context &c = ctx();
     if (c.nl.nz <= 0) return kOK;
-    constexpr int NBB = 8;
-    const long nblk = (long)c.nl.nz * ((nb + NBB - 1) / NBB);
-    const T *val = sizeof(T) == 8 ? (const T *)c.nl.row_val64.p : (const T *)c.nl.row_val32.p;
-    hipLaunchKernelGGL((k_project<T, NBB>), dim3(nblk), dim3(256), 0, c.stream, in, c.g.ML, nb,
-                       c.nl.nz, c.nl.row_ptr.as<int>(), c.nl.row_idx.as<int>(), val,
-                       c.nl.scale64.as<double>(), coef, band_active);
+    switch (c.nl.gp) {
+    case 1: launch_nl_project<T, 1>(in, nb, coef, band_active); break;
+    case 2: launch_nl_project<T, 2>(in, nb, coef, band_active); break;
+    case 4: launch_nl_project<T, 4>(in, nb, coef, band_active); break;
+    case 8: launch_nl_project<T, 8>(in, nb, coef, band_active); break;
+    case 16: launch_nl_project<T, 16>(in, nb, coef, band_active); break;
+    default: return set_error(kErrState, "projector group width %d", c.nl.gp);
+    }
     RS_HIP(hipGetLastError());
     return kOK;
 }
@@ -368,21 +526,15 @@ template <class T>
 int nl_apply(T *out, int nb, const T *coef, const int *band_active)
 {
     Context &c = ctx();
-    NonlocalTables &nl = c.nl;
-    if (nl.nz <= 0 || nl.ncov <= 0) return kOK;
-    constexpr int NBB = sizeof(T) == 8 ? 8 : 16;
-    const int nblk = (nl.ncov + 255) / 256;
-    // enough workgroups for 256 CUs: split the bands into groups
-    int ngroups = std::max(1, std::min((nb + NBB - 1) / NBB, (2048 + nblk - 1) / nblk));
-    int bpg = (nb + ngroups - 1) / ngroups;
-    bpg = (bpg + NBB - 1) / NBB * NBB;
-    ngroups = (nb + bpg - 1) / bpg;
-    const size_t lds = (((size_t)nl.max_blk_entries * sizeof(int) + 15) & ~(size_t)15) +
-                       (size_t)nl.max_blk_entries * sizeof(T);
-    const T *cval = sizeof(T) == 8 ? (const T *)nl.col_val64.p : (const T *)nl.col_val32.p;
-    hipLaunchKernelGGL((k_nl_apply<T, NBB>), dim3(nblk, ngroups), dim3(256), lds, c.stream, out,
-                       c.g.ML, nb, bpg, nl.cov_pts.as<int>(), nl.cov_ptr.as<int>(), nl.ncov,
-                       nl.col_lma.as<int>(), cval, coef, band_active);
+    if (c.nl.nz <= 0 || c.nl.npts <= 0) return kOK;
+    switch (c.nl.gp) {
+    case 1: launch_nl_apply<T, 1>(out, nb, coef, band_active); break;
+    case 2: launch_nl_apply<T, 2>(out, nb, coef, band_active); break;
+    case 4: launch_nl_apply<T, 4>(out, nb, coef, band_active); break;
+    case 8: launch_nl_apply<T, 8>(out, nb, coef, band_active); break;
+    case 16: launch_nl_apply<T, 16>(out, nb, coef, band_active); break;
+    default: return set_error(kErrState, "projector group width %d", c.nl.gp);
+    }
     RS_HIP(hipGetLastError());
     return kOK;
 }

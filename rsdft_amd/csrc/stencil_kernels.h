@@ -23,172 +23,60 @@ __device__ __forceinline__ long xcd_work_index(long L, long total) {
     return (L & 7) * per + (L >> 3);
 }
 
+// Prefetch distance (planes) of the z-march: (PF+1) must divide 2*MD so the
+// register rings of length PF+1 and 2*MD+1+PF wrap together.
+template <int MD>
+constexpr int stencil_pf() { return MD % 3 == 0 ? 2 : ((2 * MD) % 4 == 0 ? 3 : 1); }
+// LDS x pad: MD rounded up to whole chunks of PX points
+template <int MD, int PX>
+constexpr int stencil_pad() { return (MD + PX - 1) / PX * PX; }
+
+constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
+constexpr int clcm(int a, int b) { return a / cgcd(a, b) * b; }
+
+template <class T, int PX>
+struct alignas(sizeof(T) * PX) Chunk {
+    T v[PX];
+};
+
 // ---------------------------------------------------------------------------
-// Md-order hamiltonian stencil with fused epilogue
+// Md-order hamiltonian stencil: out = [kin] -1/2 lap(in) + [loc] V*in (+ out if ACC)
 //
-// Thread t of a workgroup owns row r = t / tpr of the tile and the PX points
-// x = c + tpr*px (c = t % tpr), so consecutive lanes touch consecutive words
-// in LDS (no bank conflicts) and in HBM (coalesced).  The z queue holds
-// 2*MD+1+PF planes: plane z+MD+PF is requested PF steps before it is needed,
-// and the y-halo rows of plane z+1 are requested one step ahead into
-// registers, so no load latency sits between two barriers.
+// A workgroup owns a (ty rows) x (tz planes) tile of one band and marches it
+// in z.  Thread t owns the PX consecutive x points of chunk c = t % tpr in row
+// r = t / tpr, moved as one vector load/store.  z neighbours come from a
+// register queue of 2*MD+1+PF planes; x/y neighbours from a double-buffered
+// LDS copy of the current plane (rows y0-MD .. y0+ty+MD, x wrapped into pads),
+// so every step needs one barrier.  The queue, the y-halo rows, Vloc and the
+// accumulated output are rings indexed by the unrolled step u: a load issued
+// PF planes ahead is first read PF steps later.  HM = halo chunks per thread
+// (ceil(2*MD / ty)).
 // ---------------------------------------------------------------------------
-template <class T, int MD, int PX, int PF>
-__global__ void __launch_bounds__(512, 4)
-k_hpsi(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, int N3,
-       int ty, int tz, int tpr, int tiles, int nbands, StencilCoef<T, MD> cf,
-       const T *__restrict__ vloc, int flags, const int *__restrict__ band_active)
+template <class T, int MD, int PX, int HM, int FL, int NT = 512, int HE = 0>
+__global__ void __launch_bounds__(NT)
+k_hpsi2(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, int N3, int ty,
+        int tz, int tiles, int nbands, StencilCoef<T, MD> cf, const T *__restrict__ vloc,
+        const int *__restrict__ band_active)
 {
-    constexpr int QD = 2 * MD + 1 + PF;
-    constexpr int HMAX = 2;   // halo items per thread (ty >= MD guaranteed by the host)
-    extern __shared__ unsigned char smem_raw[];
-    T *L = reinterpret_cast<T *>(smem_raw);
-    const int W = N1 + 2 * MD;            // LDS row: [MD wrapped | N1 | MD wrapped]
-    // tiles of one band go to one XCD back to back, so their shared y/z halo
-    // planes are served by that XCD's L2
-    const long total = (long)tiles * nbands;
-    const long wk = xcd_work_index(blockIdx.x, total);
-    if (wk >= total) return;
-    const int band = (int)(wk / tiles);
-    const int tile = (int)(wk - (long)band * tiles);
-    if (band_active && !band_active[band]) return;
-    const int nty = N2 / ty;
-    const int y0 = (tile % nty) * ty;
-    const int z0 = (tile / nty) * tz;
-    const int tid = threadIdx.x;
-    const int nthr = blockDim.x;
-    const int r = tid / tpr;
-    const int cx0 = tid - r * tpr;
-    const int yrow = y0 + r;
-    const long plane = (long)N1 * N2;
-    const T *src = in + (long)band * ML;
-    T *dst = out + (long)band * ML;
-    const bool kin = flags & kKin, loc = flags & kLoc, acc_out = flags & kAcc;
-    const long rowoff = (long)yrow * N1 + cx0;
-
-    // halo items of this thread: (LDS row base, global row offset)
-    const int nhalo = 2 * MD * tpr;
-    int h_lrow[HMAX], h_goff[HMAX], h_x[HMAX];
-    bool h_on[HMAX];
-#pragma unroll
-    for (int k = 0; k < HMAX; ++k) {
-        const int hr = tid + k * nthr;
-        h_on[k] = hr < nhalo;
-        const int hh = h_on[k] ? hr / tpr : 0;
-        const int c = hr - hh * tpr;
-        int yy = hh < MD ? y0 - MD + hh : y0 + ty + hh - MD;
-        yy = wrapi(yy, N2);
-        h_lrow[k] = (hh < MD ? hh : ty + hh) * W + MD + c;
-        h_goff[k] = yy * N1 + c;
-        h_x[k] = c;
-    }
-
-    T q[QD][PX];
-#pragma unroll
-    for (int k = 0; k < QD - 1; ++k) {
-        const int zz = wrapi(z0 - MD + k, N3);
-        const T *p = src + zz * plane + rowoff;
-#pragma unroll
-        for (int px = 0; px < PX; ++px) q[k][px] = p[px * tpr];
-    }
-    T hb[HMAX][PX];
-#pragma unroll
-    for (int k = 0; k < HMAX; ++k) {
-        if (!h_on[k]) continue;
-        const T *p = src + (long)z0 * plane + h_goff[k];
-#pragma unroll
-        for (int px = 0; px < PX; ++px) hb[k][px] = p[px * tpr];
-    }
-    T *Lrow = L + (r + MD) * W + MD + cx0;
-    for (int z = z0; z < z0 + tz; ++z) {
-        {   // request plane z+MD+PF
-            int zn = z + MD + PF;
-            if (zn >= N3) zn -= N3;
-            const T *p = src + zn * plane + rowoff;
-#pragma unroll
-            for (int px = 0; px < PX; ++px) q[QD - 1][px] = p[px * tpr];
-        }
-        // stage plane z: own row, the y halo rows held in registers, x pads
-#pragma unroll
-        for (int px = 0; px < PX; ++px) {
-            Lrow[px * tpr] = q[MD][px];
-            const int x = cx0 + px * tpr;
-            if (x < MD || x >= N1 - MD) Lrow[px * tpr + (x < MD ? N1 : -N1)] = q[MD][px];
-        }
-#pragma unroll
-        for (int k = 0; k < HMAX; ++k) {
-            if (!h_on[k]) continue;
-#pragma unroll
-            for (int px = 0; px < PX; ++px) {
-                L[h_lrow[k] + px * tpr] = hb[k][px];
-                const int x = h_x[k] + px * tpr;
-                if (x < MD || x >= N1 - MD)
-                    L[h_lrow[k] + px * tpr + (x < MD ? N1 : -N1)] = hb[k][px];
-            }
-        }
-        __syncthreads();
-        // request the halo rows of plane z+1
-        {
-            int zn = z + 1;
-            if (zn >= N3) zn -= N3;
-#pragma unroll
-            for (int k = 0; k < HMAX; ++k) {
-                if (!h_on[k]) continue;
-                const T *p = src + zn * plane + h_goff[k];
-#pragma unroll
-                for (int px = 0; px < PX; ++px) hb[k][px] = p[px * tpr];
-            }
-        }
-#pragma unroll
-        for (int px = 0; px < PX; ++px) {
-            const T *lr = Lrow + px * tpr;
-            T acc = T(0);
-            if (kin) {
-                acc = cf.c0 * q[MD][px];
-#pragma unroll
-                for (int m = 1; m <= MD; ++m)
-                    acc = acc + cf.cx[m - 1] * (lr[-m] + lr[m])
-                              + cf.cy[m - 1] * (lr[-m * W] + lr[m * W])
-                              + cf.cz[m - 1] * (q[MD - m][px] + q[MD + m][px]);
-            }
-            const long gi = (long)z * plane + rowoff + px * tpr;
-            if (loc) acc = acc + vloc[gi] * q[MD][px];
-            if (acc_out) acc = dst[gi] + acc;
-            dst[gi] = acc;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < QD - 1; ++k) {
-#pragma unroll
-            for (int px = 0; px < PX; ++px) q[k][px] = q[k + 1][px];
-        }
-    }
-}
-
-}  // namespace rsdft
-
-namespace rsdft {
-
-// ---------------------------------------------------------------------------
-// Software-pipelined z-march.  The z queue, the y-halo rows and Vloc are
-// rings indexed by the unrolled step u (compile-time slots, no register
-// moves), so a load issued PF planes ahead is first read PF steps later and
-// its latency overlaps PF steps of work.  QD = 2*MD+1+PF must be a multiple
-// of PF+1 so every ring returns to slot 0 after one unrolled block.
-// HM = y-halo rows per thread (ceil(2*MD/ty)).
-// ---------------------------------------------------------------------------
-template <class T, int MD, int PX, int PF, int HM>
-__global__ void __launch_bounds__(512)
-k_hpsi_pipe(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, int N3,
-            int ty, int tz, int tpr, int tiles, int nbands, StencilCoef<T, MD> cf,
-            const T *__restrict__ vloc, int flags, const int *__restrict__ band_active)
-{
+    using C = Chunk<T, PX>;
+    constexpr int PF = stencil_pf<MD>();
+    constexpr int P = stencil_pad<MD, PX>();
+    constexpr int NC = P / PX;
     constexpr int QD = 2 * MD + 1 + PF;
     constexpr int RG = PF + 1;
-    static_assert(QD % RG == 0, "queue length must be a multiple of the prefetch ring");
+    static_assert(QD % RG == 0, "ring lengths must wrap together");
+    // y-halo rows of plane p are requested at step p - HD; HE puts that at the
+    // step where the neighbouring tiles request the same rows as their own
+    // (p - MD - PF), so the second request is an L2 hit
+    constexpr int HD = HE ? MD + PF : PF;
+    constexpr int HR = HD + 1;
+    constexpr int UL = clcm(QD, HR);
+    constexpr bool KIN = (FL & kKin) != 0, LOC = (FL & kLoc) != 0, ACC = (FL & kAcc) != 0;
     extern __shared__ unsigned char smem_raw[];
-    T *L = reinterpret_cast<T *>(smem_raw);
-    const int W = N1 + 2 * MD;
+    C *L = reinterpret_cast<C *>(smem_raw);
+    const int tpr = N1 / PX;
+    const int WC = tpr + 2 * NC;
+    const int LP = (ty + 2 * MD) * WC;
     const long total = (long)tiles * nbands;
     const long wk = xcd_work_index(blockIdx.x, total);
     if (wk >= total) return;
@@ -200,115 +88,264 @@ k_hpsi_pipe(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int 
     const int z0 = (tile / nty) * tz;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int r = tid / tpr;
-    const int cx0 = tid - r * tpr;
-    const int yrow = y0 + r;
-    const long plane = (long)N1 * N2;
-    const T *src = in + (long)band * ML;
-    T *dst = out + (long)band * ML;
-    const bool kin = flags & kKin, loc = flags & kLoc, acc_out = flags & kAcc;
-    const long rowoff = (long)yrow * N1 + cx0;
+    const int c = tid - r * tpr;
+    const long planeC = (long)tpr * N2;
+    const long rowC = (long)(y0 + r) * tpr + c;
+    const C *src = reinterpret_cast<const C *>(in + (long)band * ML);
+    C *dst = reinterpret_cast<C *>(out + (long)band * ML);
+    const C *vsrc = reinterpret_cast<const C *>(vloc);
 
     const int nhalo = 2 * MD * tpr;
-    int h_lrow[HM], h_goff[HM], h_x[HM];
+    int h_l[HM], h_g[HM], h_c[HM];
     bool h_on[HM];
 #pragma unroll
     for (int k = 0; k < HM; ++k) {
         const int hr = tid + k * nthr;
         h_on[k] = hr < nhalo;
         const int hh = h_on[k] ? hr / tpr : 0;
-        const int c = hr - hh * tpr;
-        int yy = hh < MD ? y0 - MD + hh : y0 + ty + hh - MD;
-        yy = wrapi(yy, N2);
-        h_lrow[k] = (hh < MD ? hh : ty + hh) * W + MD + c;
-        h_goff[k] = yy * N1 + c;
-        h_x[k] = c;
+        const int hc = hr - hh * tpr;
+        const int yy = wrapi(hh < MD ? y0 - MD + hh : y0 + ty + hh - MD, N2);
+        h_l[k] = (hh < MD ? hh : ty + hh) * WC + NC + hc;
+        h_g[k] = yy * tpr + hc;
+        h_c[k] = hc;
     }
 
-    T q[QD][PX], hb[RG][HM][PX], vq[RG][PX];
-    // prologue: planes z0-MD .. z0+MD+PF-1 into queue slots 0..QD-2; halo/V planes z0..z0+PF-1
+    C q[QD], hb[HR][HM], vq[RG], dq[RG];
+#pragma unroll
+    for (int k = 0; k < QD - 1; ++k) q[k] = src[wrapi(z0 - MD + k, N3) * planeC + rowC];
+#pragma unroll
+    for (int k = 0; k < HD; ++k) {
+        const long zp = wrapi(z0 + k, N3) * planeC;
+#pragma unroll
+        for (int h = 0; h < HM; ++h)
+            if (h_on[h]) hb[k][h] = src[zp + h_g[h]];
+    }
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+        const long zp = wrapi(z0 + k, N3) * planeC;
+        if (LOC) vq[k] = vsrc[zp + rowC];
+        if (ACC && k < tz) dq[k] = dst[zp + rowC];
+    }
+    const int lrow = (r + MD) * WC + NC + c;
+    for (int zb = z0; zb < z0 + tz; zb += UL) {
+#pragma unroll
+        for (int u = 0; u < UL; ++u) {
+            const int z = zb + u;
+            if (z >= z0 + tz) break;
+            C *Lb = L + ((z - z0) & 1) * LP;
+            {   // prefetch: queue plane z+MD+PF; halo of plane z+HD; V and out of plane z+PF
+                // (offsets <= 2*MD+1 <= N3: one conditional subtract wraps them)
+                const int zq = z + MD + PF - (z + MD + PF >= N3 ? N3 : 0);
+                const int zh = z + HD - (z + HD >= N3 ? N3 : 0);
+                const int zv = z + PF - (z + PF >= N3 ? N3 : 0);
+                q[(u + QD - 1) % QD] = src[zq * planeC + rowC];
+#pragma unroll
+                for (int h = 0; h < HM; ++h)
+                    if (h_on[h]) hb[(u + HD) % HR][h] = src[zh * planeC + h_g[h]];
+                if (LOC) vq[(u + PF) % RG] = vsrc[zv * planeC + rowC];
+                if (ACC && z + PF < z0 + tz) dq[(u + PF) % RG] = dst[zv * planeC + rowC];
+            }
+            const C cen = q[(u + MD) % QD];
+            Lb[lrow] = cen;
+            if (c < NC) Lb[lrow + tpr] = cen;
+            if (c >= tpr - NC) Lb[lrow - tpr] = cen;
+#pragma unroll
+            for (int h = 0; h < HM; ++h) {
+                if (!h_on[h]) continue;
+                const C v = hb[u % HR][h];
+                Lb[h_l[h]] = v;
+                if (h_c[h] < NC) Lb[h_l[h] + tpr] = v;
+                if (h_c[h] >= tpr - NC) Lb[h_l[h] - tpr] = v;
+            }
+            __syncthreads();
+            C o;
+            if (KIN) {
+                C xw[2 * NC], yw[2 * MD];
+#pragma unroll
+                for (int j = 0; j < NC; ++j) {
+                    xw[j] = Lb[lrow - NC + j];
+                    xw[NC + j] = Lb[lrow + 1 + j];
+                }
+#pragma unroll
+                for (int m = 0; m < MD; ++m) {
+                    yw[m] = Lb[lrow - (m + 1) * WC];
+                    yw[MD + m] = Lb[lrow + (m + 1) * WC];
+                }
+#pragma unroll
+                for (int px = 0; px < PX; ++px) {
+                    T acc = cf.c0 * cen.v[px];
+#pragma unroll
+                    for (int m = 1; m <= MD; ++m) {
+                        const int lo = px - m + P, hi = px + m - PX;   // window offsets
+                        const T xl = px - m >= 0 ? cen.v[px - m] : xw[lo / PX].v[lo % PX];
+                        const T xr = px + m < PX ? cen.v[px + m] : xw[NC + hi / PX].v[hi % PX];
+                        acc = acc + cf.cx[m - 1] * (xl + xr)
+                                  + cf.cy[m - 1] * (yw[m - 1].v[px] + yw[MD + m - 1].v[px])
+                                  + cf.cz[m - 1] * (q[(u + MD - m) % QD].v[px] +
+                                                    q[(u + MD + m) % QD].v[px]);
+                    }
+                    o.v[px] = acc;
+                }
+            } else {
+#pragma unroll
+                for (int px = 0; px < PX; ++px) o.v[px] = T(0);
+            }
+#pragma unroll
+            for (int px = 0; px < PX; ++px) {
+                if (LOC) o.v[px] = o.v[px] + vq[u % RG].v[px] * cen.v[px];
+                if (ACC) o.v[px] = dq[u % RG].v[px] + o.v[px];
+            }
+            dst[(long)z * planeC + rowC] = o;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Strided variant: thread owns the PX points x = c + k*tpr (k < PX) of row r,
+// so lanes touch consecutive words in every scalar LDS and HBM access.  Same
+// z-march, rings and single barrier as the chunked kernel above.
+// ---------------------------------------------------------------------------
+template <class T, int MD, int PX, int HM, int FL, int LAZY = 0>
+__global__ void __launch_bounds__(512)
+k_hpsi_s(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, int N3, int ty,
+         int tz, int tiles, int nbands, StencilCoef<T, MD> cf, const T *__restrict__ vloc,
+         const int *__restrict__ band_active)
+{
+    constexpr int PF = stencil_pf<MD>();
+    constexpr int QD = 2 * MD + 1 + PF;
+    constexpr int RG = PF + 1;
+    static_assert(QD % RG == 0, "ring lengths must wrap together");
+    constexpr bool KIN = (FL & kKin) != 0, LOC = (FL & kLoc) != 0, ACC = (FL & kAcc) != 0;
+    extern __shared__ unsigned char smem_raw[];
+    T *L = reinterpret_cast<T *>(smem_raw);
+    const int tpr = N1 / PX;
+    const int W = N1 + 2 * MD;
+    const int LP = (ty + 2 * MD) * W;
+    const long total = (long)tiles * nbands;
+    const long wk = xcd_work_index(blockIdx.x, total);
+    if (wk >= total) return;
+    const int band = (int)(wk / tiles);
+    const int tile = (int)(wk - (long)band * tiles);
+    if (band_active && !band_active[band]) return;
+    const int nty = N2 / ty;
+    const int y0 = (tile % nty) * ty;
+    const int z0 = (tile / nty) * tz;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int r = tid / tpr;
+    const int c = tid - r * tpr;
+    const long plane = (long)N1 * N2;
+    const long rowoff = (long)(y0 + r) * N1 + c;
+    const T *src = in + (long)band * ML;
+    T *dst = out + (long)band * ML;
+
+    const int nhalo = 2 * MD * tpr;
+    int h_l[HM], h_g[HM], h_c[HM];
+    bool h_on[HM];
+#pragma unroll
+    for (int k = 0; k < HM; ++k) {
+        const int hr = tid + k * nthr;
+        h_on[k] = hr < nhalo;
+        const int hh = h_on[k] ? hr / tpr : 0;
+        const int hc = hr - hh * tpr;
+        const int yy = wrapi(hh < MD ? y0 - MD + hh : y0 + ty + hh - MD, N2);
+        h_l[k] = (hh < MD ? hh : ty + hh) * W + MD + hc;
+        h_g[k] = yy * N1 + hc;
+        h_c[k] = hc;
+    }
+
+    T q[QD][PX], hb[RG][HM][PX], vq[RG][PX], dq[RG][PX];
 #pragma unroll
     for (int k = 0; k < QD - 1; ++k) {
-        const T *p = src + (long)wrapi(z0 - MD + k, N3) * plane + rowoff;
+        const T *p = src + wrapi(z0 - MD + k, N3) * plane + rowoff;
 #pragma unroll
         for (int px = 0; px < PX; ++px) q[k][px] = p[px * tpr];
     }
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
-        const long zp = (long)wrapi(z0 + k, N3) * plane;
+        const long zp = wrapi(z0 + k, N3) * plane;
 #pragma unroll
-        for (int h = 0; h < HM; ++h) {
-            if (!h_on[h]) continue;
+        for (int px = 0; px < PX; ++px) {
 #pragma unroll
-            for (int px = 0; px < PX; ++px) hb[k][h][px] = src[zp + h_goff[h] + px * tpr];
-        }
-        if (loc) {
-#pragma unroll
-            for (int px = 0; px < PX; ++px) vq[k][px] = vloc[zp + rowoff + px * tpr];
+            for (int h = 0; h < HM; ++h)
+                if (h_on[h]) hb[k][h][px] = src[zp + h_g[h] + px * tpr];
+            if (LOC) vq[k][px] = vloc[zp + rowoff + px * tpr];
+            if (ACC && k < tz) dq[k][px] = dst[zp + rowoff + px * tpr];
         }
     }
-    T *Lrow = L + (r + MD) * W + MD + cx0;
+    const int lrow = (r + MD) * W + MD + c;
     for (int zb = z0; zb < z0 + tz; zb += QD) {
 #pragma unroll
         for (int u = 0; u < QD; ++u) {
             const int z = zb + u;
             if (z >= z0 + tz) break;
-            // prefetch plane z+MD+PF of the queue, halo rows and V of plane z+PF
+            T *Lb = L + ((z - z0) & 1) * LP;
             {
-                const T *p = src + (long)wrapi(z + MD + PF, N3) * plane + rowoff;
+                const int zq = z + MD + PF - (z + MD + PF >= N3 ? N3 : 0);
+                const int zv = z + PF - (z + PF >= N3 ? N3 : 0);
+                const T *p = src + zq * plane + rowoff;
+                const long zp = zv * plane;
 #pragma unroll
-                for (int px = 0; px < PX; ++px) q[(u + QD - 1) % QD][px] = p[px * tpr];
-                const long zp = (long)wrapi(z + PF, N3) * plane;
+                for (int px = 0; px < PX; ++px) {
+                    q[(u + QD - 1) % QD][px] = p[px * tpr];
 #pragma unroll
-                for (int h = 0; h < HM; ++h) {
-                    if (!h_on[h]) continue;
-#pragma unroll
-                    for (int px = 0; px < PX; ++px)
-                        hb[(u + PF) % RG][h][px] = src[zp + h_goff[h] + px * tpr];
-                }
-                if (loc) {
-#pragma unroll
-                    for (int px = 0; px < PX; ++px) vq[(u + PF) % RG][px] = vloc[zp + rowoff + px * tpr];
+                    for (int h = 0; h < HM; ++h)
+                        if (h_on[h]) hb[(u + PF) % RG][h][px] = src[zp + h_g[h] + px * tpr];
+                    if (LOC) vq[(u + PF) % RG][px] = vloc[zp + rowoff + px * tpr];
+                    if (ACC && z + PF < z0 + tz) dq[(u + PF) % RG][px] = dst[zp + rowoff + px * tpr];
                 }
             }
-            // stage plane z
 #pragma unroll
             for (int px = 0; px < PX; ++px) {
                 const T v = q[(u + MD) % QD][px];
-                Lrow[px * tpr] = v;
-                const int x = cx0 + px * tpr;
-                if (x < MD || x >= N1 - MD) Lrow[px * tpr + (x < MD ? N1 : -N1)] = v;
-            }
+                const int x = c + px * tpr;
+                Lb[lrow + px * tpr] = v;
+                if (x < MD) Lb[lrow + px * tpr + N1] = v;
+                if (x >= N1 - MD) Lb[lrow + px * tpr - N1] = v;
 #pragma unroll
-            for (int h = 0; h < HM; ++h) {
-                if (!h_on[h]) continue;
-#pragma unroll
-                for (int px = 0; px < PX; ++px) {
-                    const T v = hb[u % RG][h][px];
-                    L[h_lrow[h] + px * tpr] = v;
-                    const int x = h_x[h] + px * tpr;
-                    if (x < MD || x >= N1 - MD) L[h_lrow[h] + px * tpr + (x < MD ? N1 : -N1)] = v;
+                for (int h = 0; h < HM; ++h) {
+                    if (!h_on[h]) continue;
+                    const T w = hb[u % RG][h][px];
+                    const int xh = h_c[h] + px * tpr;
+                    Lb[h_l[h] + px * tpr] = w;
+                    if (xh < MD) Lb[h_l[h] + px * tpr + N1] = w;
+                    if (xh >= N1 - MD) Lb[h_l[h] + px * tpr - N1] = w;
                 }
             }
             __syncthreads();
 #pragma unroll
             for (int px = 0; px < PX; ++px) {
-                const T *lr = Lrow + px * tpr;
                 const T cen = q[(u + MD) % QD][px];
+                const T *lr = Lb + lrow + px * tpr;
                 T acc = T(0);
-                if (kin) {
+                if (KIN && LAZY) {
+                    // neighbours read as they are summed: fewer live registers
                     acc = cf.c0 * cen;
 #pragma unroll
                     for (int m = 1; m <= MD; ++m)
                         acc = acc + cf.cx[m - 1] * (lr[-m] + lr[m])
                                   + cf.cy[m - 1] * (lr[-m * W] + lr[m * W])
                                   + cf.cz[m - 1] * (q[(u + MD - m) % QD][px] + q[(u + MD + m) % QD][px]);
+                } else if (KIN) {
+                    T xw[2 * MD], yw[2 * MD];
+#pragma unroll
+                    for (int m = 0; m < MD; ++m) {
+                        xw[m] = lr[-(m + 1)];
+                        xw[MD + m] = lr[m + 1];
+                        yw[m] = lr[-(m + 1) * W];
+                        yw[MD + m] = lr[(m + 1) * W];
+                    }
+                    acc = cf.c0 * cen;
+#pragma unroll
+                    for (int m = 0; m < MD; ++m)
+                        acc = acc + cf.cx[m] * (xw[m] + xw[MD + m])
+                                  + cf.cy[m] * (yw[m] + yw[MD + m])
+                                  + cf.cz[m] * (q[(u + MD - 1 - m) % QD][px] +
+                                                q[(u + MD + 1 + m) % QD][px]);
                 }
-                const long gi = (long)z * plane + rowoff + px * tpr;
-                if (loc) acc = acc + vq[u % RG][px] * cen;
-                if (acc_out) acc = dst[gi] + acc;
-                dst[gi] = acc;
+                if (LOC) acc = acc + vq[u % RG][px] * cen;
+                if (ACC) acc = dq[u % RG][px] + acc;
+                dst[(long)z * plane + rowoff + px * tpr] = acc;
             }
-            __syncthreads();
         }
     }
 }
