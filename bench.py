@@ -13,6 +13,12 @@ with HIP events on the library's stream.  The CPU baseline is the oracle's
 OpenMP FP64 restatement on a bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps 9] [--warmup 1]
+
+HBM traffic of the roofline kernel comes from two rocprofv3 counter runs of
+`python bench.py --traffic-probe` (FETCH_SIZE, then WRITE_SIZE), each scaled by
+what the same counter reads for the library's plain copy of all bands; they are
+summarised with `python bench.py --pmc-summary FETCH_DIR WRITE_DIR` into
+profiles/hpsi_traffic_si216.json, which this script reads when present.
 """
 from __future__ import annotations
 
@@ -34,11 +40,85 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def hpsi_stencil_bytes(sysm, nb, s):
-    """Algorithmic HBM bytes of one all-band launch of the stencil kernel
-    (kinetic + local): psi read + H psi write per band and Vloc once."""
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hpsi_traffic_si216.json")
+
+
+def covered_points(sysm):
+    """Grid points inside some projector sphere."""
+    P = sysm.proj
+    pts = [P.JJP[l, :P.MJJ[l]] for l in range(P.nzlma)]
+    return int(np.unique(np.concatenate(pts)).size) if pts else 0
+
+
+def hpsi_stencil_bytes(sysm, nb, s, fused_nl):
+    """Algorithmic HBM bytes of one all-band launch of the stencil kernel:
+    psi read + H psi write per band and Vloc once; with the fused nonlocal
+    epilogue also the compact nonlocal term of every band and the covered-point
+    index once."""
     ML = sysm.grid.ML
-    return 2 * ML * nb * s + ML * s
+    b = 2 * ML * nb * s + ML * s
+    if fused_nl:
+        b += covered_points(sysm) * nb * s + ML * 4
+    return b
+
+
+def pmc_summary(fetch_dir, write_dir, out_path):
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the FP32 hamiltonian stencil
+    kernel, scaled by what the same counters read for the plain copy of all bands."""
+    import csv
+    import glob
+    import re
+    from rsdft_amd.system import make_config
+
+    def load(d, counter):
+        vals = {}
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                if r["Counter_Name"] == counter:
+                    vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return vals
+
+    sysm = make_config("si216")
+    ML, nb = sysm.grid.ML, sysm.nband
+    copy_bytes = ML * nb * 4            # read (and write) of one FP32 copy of all bands
+    res = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        vals = load(d, counter)
+        kern = [k for k in vals if re.search(r"k_hpsi2<float, \d+, \d+, \d+, 3,", k)]
+        copy = [k for k in vals if "k_copy_bands<float>" in k]
+        if not kern or not copy:
+            raise SystemExit(f"{counter}: kernels not found in {d}")
+        kv = float(np.mean(vals[kern[0]])) * 1024
+        cv = float(np.mean(vals[copy[0]])) * 1024
+        scale = copy_bytes / cv
+        res[counter] = {"kernel": kern[0], "kernel_bytes_raw": kv, "copy_bytes_raw": cv,
+                        "copy_bytes_algorithmic": copy_bytes, "scale": scale,
+                        "kernel_bytes": kv * scale, "dispatches": len(vals[kern[0]])}
+    res["traffic_bytes"] = res["FETCH_SIZE"]["kernel_bytes"] + res["WRITE_SIZE"]["kernel_bytes"]
+    res["algorithmic_bytes"] = hpsi_stencil_bytes(sysm, nb, 4, False)
+    res["commands"] = [
+        "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc FETCH_SIZE "
+        "-d <repo>/gpurun_out/pmc_fetch -o run -f csv -- python <repo>/bench.py --traffic-probe",
+        "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc WRITE_SIZE "
+        "-d <repo>/gpurun_out/pmc_write -o run -f csv -- python <repo>/bench.py --traffic-probe",
+        "python bench.py --pmc-summary gpurun_out/pmc_fetch gpurun_out/pmc_write",
+    ]
+    with open(out_path, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res, indent=1))
+
+
+def traffic_probe():
+    """A few FP32 all-band applications of every hamiltonian piece and the copy
+    (the counter runs profile this)."""
+    from rsdft_amd.system import make_config
+    from rsdft_amd.gpu import RsdftGpu, PREC_MIXED_FP32
+    sysm = make_config("si216")
+    x = sysm.psi0()
+    dev = RsdftGpu(sysm, precision=PREC_MIXED_FP32)
+    dev.attach(x)
+    t = dev.test_hpsi(3)
+    log(f"probe hpsi ms: {t}")
 
 
 def cpu_baseline(sysm, x0, nsample):
@@ -84,7 +164,15 @@ def main():
     ap.add_argument("--hpsi-reps", type=int, default=20)
     ap.add_argument("--gemm", choices=["fp32", "bf16"], default="fp32",
                     help="GEMM precision of the mixed path")
+    ap.add_argument("--traffic-probe", action="store_true")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], TRAFFIC_FILE)
+        return
+    if args.traffic_probe:
+        traffic_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the path runs on one MI355X; multi-GPU sharding is design-only)", file=sys.stderr)
@@ -153,10 +241,15 @@ def main():
 
     # ---- roofline of the dominant kernel (fused hpsi stencil, FP32) ---------
     t_kernel_ms = float(hpm[4])
-    B = hpsi_stencil_bytes(sysm, nb, 4)
+    B = hpsi_stencil_bytes(sysm, nb, 4, False)
     achieved = B / (t_kernel_ms * 1e-3) / 1e9
-    B64 = hpsi_stencil_bytes(sysm, nb, 8)
+    B64 = hpsi_stencil_bytes(sysm, nb, 8, False)
     ach64 = B64 / (float(hp64[4]) * 1e-3) / 1e9
+    copy_gbs = 2 * sysm.grid.ML * nb * 4 / (float(hpm[7]) * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = float(json.load(f)["traffic_bytes"])
 
     cpu = None
     if not args.no_cpu:
@@ -188,6 +281,10 @@ def main():
             "hpsi_ms_fp32": float(hpm[0]),
             "hpsi_ms_fp64": float(hp64[0]),
             "hpsi_stencil_gbs_fp64": ach64,
+            "hpsi_parts_ms_fp32": {k: float(v) for k, v in zip(
+                ("hamiltonian", "kinetic", "localpot", "nonlocal", "stencil", "projection",
+                 "nl_stage", "copy"), hpm)},
+            "copy_gbs_fp32": copy_gbs,
             "phase_ms_mixed": {k: v / K for k, v in timm.items() if k not in ("hpsi_bands", "eig_refined")},
             "eig_refined_mixed": timm["eig_refined"],
             "phase_ms_fp64": {k: v / K for k, v in tim64.items() if k not in ("hpsi_bands", "eig_refined")},
@@ -195,12 +292,13 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_hpsi<float,6,4,2> (Md=6 FD stencil + Vloc, all 480 bands)",
+            "kernel": "k_hpsi2<float,6,2,1,kin|loc,1024> (Md=6 FD stencil + Vloc, all bands)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "algorithmic_bytes": B,
         },
         "cpu_baseline": cpu,
     }

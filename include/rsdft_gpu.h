@@ -136,10 +136,13 @@ RSDFT_API int rsdft_gpu_band_energy_grad(const double *unk, int ML0, int MB, con
                                double *E, double *dEdV, double *dEdR);
 
 /* test_hpsi2 (test_hpsi2_module.f90:16-143): times nrep applications to the
- * attached wavefunctions on the library's stream; times_ms[7] = mean ms per
+ * attached wavefunctions on the library's stream; times_ms[8] = mean ms per
  * all-band application of [0] hamiltonian, [1] op_kinetic, [2] op_localpot,
- * [3] op_nonlocal, [4] the stencil kernel (kinetic + local), [5] the
- * projection kernel, [6] the nonlocal apply kernel. */
+ * [3] op_nonlocal, [4] the stencil kernel as the hamiltonian runs it (FP32:
+ * kinetic + local + the staged nonlocal term; FP64: kinetic + local), [5] the
+ * projection kernel, [6] the nonlocal term stage (FP32: gather into the
+ * compact buffer; FP64: scatter-add), [7] a plain copy of all bands (the
+ * bandwidth reference). */
 RSDFT_API int rsdft_gpu_test_hpsi(int nrep, double *times_ms);
 
 /* Per-phase timers in milliseconds since the last reset (watch_module.f90

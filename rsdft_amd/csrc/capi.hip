@@ -361,6 +361,25 @@ int rsdft_gpu_set_projectors(int nzlma, int maxmjj, const int *MJJ, const int *J
             if (gcol[g] == col) order.push_back(g);
         nl.color_ptr.push_back((int)order.size());
     }
+    // covered points, their compact index and group memberships
+    std::vector<int> cidx(ML, -1), mcount;
+    int ncov = 0;
+    for (long k = 0; k < npts; ++k) {
+        int &ci = cidx[gidx[k]];
+        if (ci < 0) ci = -2;   // mark
+    }
+    for (long i = 0; i < ML; ++i)
+        if (cidx[i] == -2) cidx[i] = ncov++;
+    std::vector<int> mptr(ncov + 1, 0);
+    for (long k = 0; k < npts; ++k) mptr[cidx[gidx[k]] + 1]++;
+    for (int q = 0; q < ncov; ++q) mptr[q + 1] += mptr[q];
+    std::vector<int> mpos(npts), mg(npts), mfill(mptr.begin(), mptr.end() - 1);
+    for (int g = 0; g < ng; ++g)   // memberships of a point in group order
+        for (int k = gptr[g]; k < gptr[g + 1]; ++k) {
+            const int e = mfill[cidx[gidx[k]]]++;
+            mpos[e] = k;
+            mg[e] = g;
+        }
     std::vector<float> gval32(gval.begin(), gval.end());
     std::vector<double> scale(nzlma);
     for (int l = 0; l < nzlma; ++l) scale[l] = (double)iuV[l] * c.g.dV;
@@ -379,6 +398,17 @@ int rsdft_gpu_set_projectors(int nzlma, int maxmjj, const int *MJJ, const int *J
         RS_HIP(hipMemcpy(nl.g_val32.p, gval32.data(), sizeof(float) * gval.size(), hipMemcpyHostToDevice));
     }
     RS_HIP(hipMemcpy(nl.g_order.p, order.data(), sizeof(int) * ng, hipMemcpyHostToDevice));
+    RS_TRY(nl.cov_index.ensure(sizeof(int) * ML));
+    RS_TRY(nl.mem_ptr.ensure(sizeof(int) * (ncov + 1)));
+    RS_TRY(nl.mem_pos.ensure(sizeof(int) * std::max<long>(npts, 1)));
+    RS_TRY(nl.mem_g.ensure(sizeof(int) * std::max<long>(npts, 1)));
+    RS_HIP(hipMemcpy(nl.cov_index.p, cidx.data(), sizeof(int) * ML, hipMemcpyHostToDevice));
+    RS_HIP(hipMemcpy(nl.mem_ptr.p, mptr.data(), sizeof(int) * (ncov + 1), hipMemcpyHostToDevice));
+    if (npts) {
+        RS_HIP(hipMemcpy(nl.mem_pos.p, mpos.data(), sizeof(int) * npts, hipMemcpyHostToDevice));
+        RS_HIP(hipMemcpy(nl.mem_g.p, mg.data(), sizeof(int) * npts, hipMemcpyHostToDevice));
+    }
+    nl.ncov = ncov;
     RS_HIP(hipMemcpy(nl.scale64.p, scale.data(), sizeof(double) * nzlma, hipMemcpyHostToDevice));
     nl.nz = nzlma;
     nl.ng = ng;
@@ -715,15 +745,15 @@ int rsdft_gpu_test_hpsi(int nrep, double *times_ms)
     Context &c = ctx();
     if (!c.host_unk) return set_error(kErrState, "attach wavefunctions first");
     // [0] hamiltonian, [1] op_kinetic, [2] op_localpot, [3] op_nonlocal,
-    // [4] stencil kernel (kinetic + local), [5] projection kernel,
-    // [6] nonlocal apply kernel
-    const int flags[7] = {kKin | kLoc | kNl, kKin, kLoc | kAcc, kNl | kAcc, -1, -2, -3};
+    // [4] stencil kernel as in the hamiltonian, [5] projection kernel,
+    // [6] nonlocal term stage, [7] plain copy of all bands
+    const int flags[8] = {kKin | kLoc | kNl, kKin, kLoc | kAcc, kNl | kAcc, -1, -2, -3, -4};
     hipEvent_t a, b;
     RS_HIP(hipEventCreate(&a));
     RS_HIP(hipEventCreate(&b));
     const long saved = c.hpsi_bands;
     RS_TRY(c.coefC.ensure((size_t)c.MB * std::max(c.nl.nz, 1) * elem_bytes(c.prec)));
-    for (int t = 0; t < 7; ++t) {
+    for (int t = 0; t < 8; ++t) {
         for (int rep = -1; rep < nrep; ++rep) {
             if (rep == 0) RS_HIP(hipEventRecord(a, c.stream));
             if (c.prec == 0) {
@@ -731,13 +761,15 @@ int rsdft_gpu_test_hpsi(int nrep, double *times_ms)
                 if (flags[t] >= 0) RS_TRY(hpsi_apply<double>(x, y, c.MB, flags[t]));
                 else if (flags[t] == -1) RS_TRY(hpsi_stencil_only<double>(x, y, c.MB));
                 else if (flags[t] == -2) RS_TRY(nl_project<double>(x, c.MB, cf));
-                else RS_TRY(nl_apply<double>(y, c.MB, cf));
+                else if (flags[t] == -3) RS_TRY(nl_stage<double>(y, c.MB, cf));
+                else RS_TRY(copy_bands<double>(x, y, c.MB));
             } else {
                 float *x = psiT<float>(), *y = c.hp.as<float>(), *cf = c.coefC.as<float>();
                 if (flags[t] >= 0) RS_TRY(hpsi_apply<float>(x, y, c.MB, flags[t]));
                 else if (flags[t] == -1) RS_TRY(hpsi_stencil_only<float>(x, y, c.MB));
                 else if (flags[t] == -2) RS_TRY(nl_project<float>(x, c.MB, cf));
-                else RS_TRY(nl_apply<float>(y, c.MB, cf));
+                else if (flags[t] == -3) RS_TRY(nl_stage<float>(y, c.MB, cf));
+                else RS_TRY(copy_bands<float>(x, y, c.MB));
             }
         }
         RS_HIP(hipEventRecord(b, c.stream));
@@ -781,6 +813,7 @@ int rsdft_gpu_finalize(void)
                      &c.mat3, &c.eval, &c.einfo, &c.evwork, &c.matT, &c.dots, &c.gemm_part, &c.opbuf, &c.pc_coef,
                      &c.cg.buf, &c.cg.flags, &c.nl.g_lma, &c.nl.g_ptr, &c.nl.g_idx, &c.nl.g_val64,
                      &c.nl.g_val32, &c.nl.g_order, &c.nl.scale64, &c.nl.d_ptr, &c.nl.d_idx, &c.nl.d_val,
+                     &c.nl.cov_index, &c.nl.mem_ptr, &c.nl.mem_pos, &c.nl.mem_g, &c.nlbuf,
                      &c.nl.amap_dev};
     for (DevMem *m : all) m->release();
     if (c.blas) rocblas_destroy_handle(c.blas);
@@ -795,6 +828,7 @@ int rsdft_gpu_finalize(void)
     c.nl.nz = 0;
     c.nl.ng = 0;
     c.nl.npts = 0;
+    c.nl.ncov = 0;
     c.nl.color_ptr.clear();
     c.nl.natom = 0;
     c.pc_mode = 1;

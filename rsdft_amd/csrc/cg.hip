@@ -173,6 +173,17 @@ __global__ void k_cg_pgg(CGView v, const double *part, int nb, int nchunk, doubl
     v.gkgk[b] = rb;
 }
 
+// the same from (Pg, g) given by the s-step preconditioner's Gram matrix
+__global__ void k_cg_pgg_gram(CGView v, const double *pgg, int nb, double dV, int first)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb || !v.active[b]) return;
+    const double rb = dV * pgg[b];
+    v.rb[b] = rb;
+    v.bk[b] = first ? 0.0 : rb / v.gkgk[b];
+    v.gkgk[b] = rb;
+}
+
 // 2x2 Rayleigh-Ritz with the swap rule and the sign fix (cg_module.f90:286-333)
 __global__ void k_cg_2x2(CGView v, const double *part, int nb, int nchunk, double dV)
 {
@@ -291,36 +302,51 @@ k_cg_newp(const T *PG, T *P, CGView v, long ML, int first)
 }
 
 // Hx = u1 Hx + u2 Hp; g = -2 (Hx - W (u1 x + u2 p)); part = sum g^2
-// (cg_module.f90:337-349)
+// (cg_module.f90:337-349).  When u1 > 1/2 the new x = u1 x + u2 p is stored
+// here already (the reject rule of cg_module.f90:360-368 almost never fires;
+// k_cg_accept undoes it then), which saves a pass over x and p.
+__device__ __forceinline__ bool cg_speculative(double u1) { return u1 > 0.5; }
+
 template <class T>
 __global__ void __launch_bounds__(kChunkThreads)
-k_cg_update(T *HX, const T *HP, const T *X, const T *P, T *G, CGView v, long ML, int nchunk,
+k_cg_update(T *HX, const T *HP, T *X, const T *P, T *G, CGView v, long ML, int nchunk,
             double *part)
 {
     BAND_LOOP_BEGIN
     if (!v.active[b]) return;
     const T u1 = (T)v.u1[b], u2 = (T)v.u2[b], w = (T)v.W[b];
+    const bool spec = cg_speculative(v.u1[b]);
     double s[1] = {0.0};
     for (long i = i0 + threadIdx.x; i < i1; i += kChunkThreads) {
         const T hx = u1 * HX[off + i] + u2 * HP[off + i];
         HX[off + i] = hx;
-        const T g = T(-2) * (hx - w * (u1 * X[off + i] + u2 * P[off + i]));
+        const T xn = u1 * X[off + i] + u2 * P[off + i];
+        const T g = T(-2) * (hx - w * xn);
         G[off + i] = g;
+        if (spec) X[off + i] = xn;
         s[0] += (double)g * (double)g;
     }
     block_sum_write<1>(s, part + (long)b * nchunk + chunk, 0);
 }
 
-// x = u1 x + u2 p unless rejected (cg_module.f90:366-368)
+// x = u1 x + u2 p unless rejected (cg_module.f90:366-368); for the bands
+// k_cg_update already moved, a rejection restores x = (x_new - u2 p) / u1
 template <class T>
 __global__ void __launch_bounds__(kChunkThreads)
 k_cg_accept(T *X, const T *P, CGView v, long ML)
 {
     BAND_LOOP_BEGIN
-    if (!v.active[b] || !v.accept[b]) return;
+    if (!v.active[b]) return;
+    const bool spec = cg_speculative(v.u1[b]);
+    if (spec == (v.accept[b] != 0)) return;
     const T u1 = (T)v.u1[b], u2 = (T)v.u2[b];
-    for (long i = i0 + threadIdx.x; i < i1; i += kChunkThreads)
-        X[off + i] = u1 * X[off + i] + u2 * P[off + i];
+    if (spec) {
+        for (long i = i0 + threadIdx.x; i < i1; i += kChunkThreads)
+            X[off + i] = (X[off + i] - u2 * P[off + i]) / u1;
+    } else {
+        for (long i = i0 + threadIdx.x; i < i1; i += kChunkThreads)
+            X[off + i] = u1 * X[off + i] + u2 * P[off + i];
+    }
 }
 
 // Pg += a pp; unless last: r -= a q, part = sum r^2
@@ -471,7 +497,9 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
     const int maxparts = std::max(nch, c.geom_pc.tiles);
     RS_TRY(c.part.ensure(std::max(sizeof(double) * (size_t)6 * nb * maxparts,
                                   sizeof(double) * precond_sstep_part_elems(nb))));
-    RS_TRY(c.pc_coef.ensure(sizeof(double) * (size_t)nb * std::max(c.mloop, 1)));
+    RS_TRY(c.pc_coef.ensure(sizeof(double) * (size_t)nb * (std::max(c.mloop, 1) + 1)));
+    // the s-step preconditioner hands (Pg, g) and the new direction over directly
+    const bool sstep = c.pc_mode == 1 && c.mloop >= 1 && c.mloop <= 4;
     double *part = c.part.as<double>();
     CGView v = view(c.cg);
     T *HX = c.hx.as<T>(), *HP = c.hp.as<T>(), *G = c.gk.as<T>(), *PG = c.pgk.as<T>(),
@@ -491,15 +519,24 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
     for (int icg = 1; icg <= c.ncg + 1; ++icg) {
         hipLaunchKernelGGL(k_cg_top, bgrid, bblk, 0, c.stream, v, nb);
         if (icg == c.ncg + 1) break;
-        RS_TRY(precond_batched<T>(G, PG, nb, act));
-        {
-            const T *a[1] = {PG}, *bb[1] = {G};
-            RS_TRY(band_dots<T>(1, a, bb, nb, 1.0, nullptr, act));
-            hipLaunchKernelGGL(k_cg_pgg, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV,
+        if (sstep) {
+            // Gram pass -> d, (Pg,g) -> rb, beta -> P = Pg + beta P in the apply pass
+            double *dcoef = c.pc_coef.as<double>(), *pgg = dcoef + (size_t)nb * c.mloop;
+            RS_TRY(precond_sstep_coeffs<T>(G, nb, act, part, dcoef, pgg));
+            hipLaunchKernelGGL(k_cg_pgg_gram, bgrid, bblk, 0, c.stream, v, pgg, nb, dV,
+                               icg == 1 ? 1 : 0);
+            RS_TRY(precond_sstep_apply<T>(G, P, nb, act, dcoef, 1, icg == 1 ? nullptr : v.bk));
+        } else {
+            RS_TRY(precond_batched<T>(G, PG, nb, act));
+            {
+                const T *a[1] = {PG}, *bb[1] = {G};
+                RS_TRY(band_dots<T>(1, a, bb, nb, 1.0, nullptr, act));
+                hipLaunchKernelGGL(k_cg_pgg, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV,
+                                   icg == 1 ? 1 : 0);
+            }
+            hipLaunchKernelGGL((k_cg_newp<T>), egrid, eblk, 0, c.stream, PG, P, v, ML,
                                icg == 1 ? 1 : 0);
         }
-        hipLaunchKernelGGL((k_cg_newp<T>), egrid, eblk, 0, c.stream, PG, P, v, ML,
-                           icg == 1 ? 1 : 0);
         RS_TRY(hpsi_apply<T>(P, HP, nb, kKin | kLoc | kNl, act));
         {
             const T *a[6] = {X, P, P, X, P, P}, *bb[6] = {X, X, P, HX, HX, HP};

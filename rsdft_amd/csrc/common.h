@@ -16,6 +16,10 @@ constexpr int kMaxMd = 8;
 // thread, FP64 two x points tpr apart (strided); workgroup size limits
 constexpr int kHpsiPx32 = 2, kHpsiPx64 = 2;
 constexpr int kHpsiThreads32 = 1024, kHpsiThreads64 = 512;
+// FP32 H psi: nonlocal term through a compact per-covered-point buffer added
+// in the stencil epilogue (true) or scatter-added after the stencil (false).
+// Measured at Si216: 1.44 ms vs 1.26 ms per all-band H psi, so off.
+constexpr bool kFuseNonlocal = false;
 
 enum Status {
     kOK = 0,
@@ -99,6 +103,12 @@ struct NonlocalTables {
     DevMem g_val64, g_val32;    // [npts*gp] uV of each slot at each point (0 outside its sphere)
     DevMem g_order;             // groups ordered by colour
     std::vector<int> color_ptr; // colour k: g_order[color_ptr[k] .. color_ptr[k+1])
+    // covered grid points (inside some sphere) in grid order: compact index
+    // of every grid point (-1 outside), and per covered point the list of
+    // (group, union position) it belongs to
+    int ncov = 0;
+    DevMem cov_index;           // [ML]
+    DevMem mem_ptr, mem_pos, mem_g;
     DevMem scale64;             // iuV(lma) * dV
     // force tables
     int natom = 0;
@@ -156,6 +166,7 @@ struct Context {
     int wf_prec = -1;
     DevMem psi, hx, hp, gk, pgk, pk, rk, ppc, qpc, tmp;
     DevMem coefC;      // projection coefficients [lma][band]
+    DevMem nlbuf;      // nonlocal term at the covered points [band][ncov]
     DevMem part;       // reduction partial sums
     DevMem stage;      // host<->device staging (FP64)
     DevMem opbuf;      // working-precision staging of the op_* entry points
@@ -186,9 +197,16 @@ template <class T>
 int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active = nullptr);
 template <class T>
 int nl_project(const T *in, int nb, T *coef, const int *band_active = nullptr);
-// the stencil kernel alone (kinetic + local, overwrite)
+// The pieces of one H psi, for timing: the stencil kernel as hpsi_apply runs
+// it (FP32: with the compact nonlocal term of the last nl_stage call fused
+// in; FP64: kinetic + local), the nonlocal term stage (FP32: gather into the
+// compact buffer; FP64: scatter-add into out), and a plain copy of nb bands.
 template <class T>
 int hpsi_stencil_only(const T *in, T *out, int nb);
+template <class T>
+int nl_stage(T *out, int nb, const T *coef);
+template <class T>
+int copy_bands(const T *in, T *out, int nb);
 // out += V_NL in from the projection coefficients coef[lma][band]
 template <class T>
 int nl_apply(T *out, int nb, const T *coef, const int *band_active = nullptr);
@@ -200,6 +218,15 @@ int precond_batched(const T *g, T *pg, int nb, const int *active);
 // s-step form of the same preconditioner (precond_sstep.hip)
 template <class T>
 int precond_sstep(const T *g, T *pg, int nb, const int *active, double *part, double *dcoef);
+// its two passes: coefficients d[b][0..mloop) and pgg[b] = (Pg, g) (may be
+// null), then out = g + sum d_k v_k, or with fuse = 1 out = that + beta out
+// (beta == null: overwrite)
+template <class T>
+int precond_sstep_coeffs(const T *g, int nb, const int *active, double *part, double *dcoef,
+                         double *pgg);
+template <class T>
+int precond_sstep_apply(const T *g, T *out, int nb, const int *active, const double *dcoef,
+                        int fuse, const double *beta);
 size_t precond_sstep_part_elems(int nb);
 
 // reductions: out[k*nb + b] = scale * sum_i a_k[b,i]*b_k[b,i], k < ndot

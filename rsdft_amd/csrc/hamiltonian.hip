@@ -182,6 +182,50 @@ k_nl_apply(T *__restrict__ out, long ML, int nb, const int *__restrict__ order, 
 }
 
 // ---------------------------------------------------------------------------
+// Nonlocal term at the covered grid points, in compact order:
+//   nlbuf[b][c] = sum_{(g, pos) containing c} sum_s uV(pos, s) coefC[lma(g, s)][b]
+// one thread per covered point (consecutive threads write consecutive words);
+// the stencil adds it in its epilogue, so the output is written once.
+// ---------------------------------------------------------------------------
+template <class T, int P, int NBB>
+__global__ void __launch_bounds__(256)
+k_nl_gather(T *__restrict__ nlbuf, int ncov, int nb, int bands_per_group,
+            const int *__restrict__ mptr, const int *__restrict__ mpos, const int *__restrict__ mg,
+            const T *__restrict__ gval, const int *__restrict__ glma, const T *__restrict__ coefC,
+            const int *__restrict__ band_active)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncov) return;
+    const int m0 = mptr[c], m1 = mptr[c + 1];
+    const int bg0 = blockIdx.y * bands_per_group;
+    const int bg1 = min(nb, bg0 + bands_per_group);
+    for (int b0 = bg0; b0 < bg1; b0 += NBB) {
+        T acc[NBB];
+#pragma unroll
+        for (int k = 0; k < NBB; ++k) acc[k] = T(0);
+        for (int m = m0; m < m1; ++m) {
+            const int g = mg[m];
+            const long pos = mpos[m];
+#pragma unroll
+            for (int s = 0; s < P; ++s) {
+                const int lma = glma[g * P + s];
+                if (lma < 0) continue;
+                const T u = gval[pos * P + s];
+                const T *cr = coefC + (long)lma * nb + b0;
+#pragma unroll
+                for (int k = 0; k < NBB; ++k)
+                    if (b0 + k < bg1) acc[k] = acc[k] + u * cr[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NBB; ++k) {
+            const int b = b0 + k;
+            if (b < bg1 && (!band_active || band_active[b])) nlbuf[(long)b * ncov + c] = acc[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // 7-point preconditioner operator of precond_cg_mat2_omp with E = 0
 // (cgpc_hprsdft_module.f90:281-348):
 //   (A f)(i) = c f(i) + sum_d c_d (f(i+e_d) + f(i-e_d)),
@@ -387,7 +431,7 @@ StencilGeom choose_hpsi_geom(int N1, int N2, int N3, int md, int px, int pad, in
 
 template <class T, int MD, int HM, int FL>
 static void launch_hpsi_k(const T *in, T *out, int nb, const int *band_active, const T *vloc,
-                          const StencilCoef<T, MD> &cf)
+                          const StencilCoef<T, MD> &cf, const T *nlbuf)
 {
     Context &c = ctx();
     const GridDesc &g = c.g;
@@ -401,12 +445,13 @@ static void launch_hpsi_k(const T *in, T *out, int nb, const int *band_active, c
     else
         hipLaunchKernelGGL((k_hpsi2<T, MD, kHpsiPx32, HM, FL, kHpsiThreads32>), grid,
                            dim3(G.threads), G.lds, c.stream, in, out, g.ML, g.N1, g.N2, g.N3, G.ty,
-                           G.tz, G.tiles, nb, cf, vloc, band_active);
+                           G.tz, G.tiles, nb, cf, vloc, band_active, c.nl.cov_index.as<int>(),
+                           nlbuf, c.nl.ncov);
 }
 
 template <class T, int MD>
 static int launch_hpsi_md(const T *in, T *out, int nb, int flags, const int *band_active,
-                          const T *vloc)
+                          const T *vloc, const T *nlbuf)
 {
     Context &c = ctx();
     const GridDesc &g = c.g;
@@ -420,11 +465,18 @@ static int launch_hpsi_md(const T *in, T *out, int nb, int flags, const int *ban
     }
     const bool h1 = G.hm == 1;
     if (flags == (kKin | kLoc)) {
-        if (h1) launch_hpsi_k<T, MD, 1, kKin | kLoc>(in, out, nb, band_active, vloc, cf);
-        else launch_hpsi_k<T, MD, 2, kKin | kLoc>(in, out, nb, band_active, vloc, cf);
+        if (h1) launch_hpsi_k<T, MD, 1, kKin | kLoc>(in, out, nb, band_active, vloc, cf, nullptr);
+        else launch_hpsi_k<T, MD, 2, kKin | kLoc>(in, out, nb, band_active, vloc, cf, nullptr);
     } else if (flags == kKin) {
-        if (h1) launch_hpsi_k<T, MD, 1, kKin>(in, out, nb, band_active, vloc, cf);
-        else launch_hpsi_k<T, MD, 2, kKin>(in, out, nb, band_active, vloc, cf);
+        if (h1) launch_hpsi_k<T, MD, 1, kKin>(in, out, nb, band_active, vloc, cf, nullptr);
+        else launch_hpsi_k<T, MD, 2, kKin>(in, out, nb, band_active, vloc, cf, nullptr);
+    } else if (flags == (kKin | kLoc | kNl)) {
+        if constexpr (sizeof(T) == 4) {
+            if (h1) launch_hpsi_k<T, MD, 1, kKin | kLoc | kNl>(in, out, nb, band_active, vloc, cf, nlbuf);
+            else launch_hpsi_k<T, MD, 2, kKin | kLoc | kNl>(in, out, nb, band_active, vloc, cf, nlbuf);
+        } else {
+            return set_error(kErrArg, "fused nonlocal epilogue is FP32 only");
+        }
     } else {
         return set_error(kErrArg, "stencil flags %d not supported", flags);
     }
@@ -448,7 +500,8 @@ k_localpot_acc(const T *__restrict__ in, T *__restrict__ out, const T *__restric
 }
 
 template <class T>
-static int launch_stencil(const T *in, T *out, int nb, int flags, const int *band_active)
+static int launch_stencil(const T *in, T *out, int nb, int flags, const int *band_active,
+                          const T *nlbuf = nullptr)
 {
     Context &c = ctx();
     const T *vloc = sizeof(T) == 8 ? (const T *)c.vloc64.p : (const T *)c.vloc32.p;
@@ -462,12 +515,12 @@ static int launch_stencil(const T *in, T *out, int nb, int flags, const int *ban
         return kOK;
     }
     switch (c.g.Md) {
-    case 1: return launch_hpsi_md<T, 1>(in, out, nb, flags, band_active, vloc);
-    case 2: return launch_hpsi_md<T, 2>(in, out, nb, flags, band_active, vloc);
-    case 3: return launch_hpsi_md<T, 3>(in, out, nb, flags, band_active, vloc);
-    case 4: return launch_hpsi_md<T, 4>(in, out, nb, flags, band_active, vloc);
-    case 6: return launch_hpsi_md<T, 6>(in, out, nb, flags, band_active, vloc);
-    case 8: return launch_hpsi_md<T, 8>(in, out, nb, flags, band_active, vloc);
+    case 1: return launch_hpsi_md<T, 1>(in, out, nb, flags, band_active, vloc, nlbuf);
+    case 2: return launch_hpsi_md<T, 2>(in, out, nb, flags, band_active, vloc, nlbuf);
+    case 3: return launch_hpsi_md<T, 3>(in, out, nb, flags, band_active, vloc, nlbuf);
+    case 4: return launch_hpsi_md<T, 4>(in, out, nb, flags, band_active, vloc, nlbuf);
+    case 6: return launch_hpsi_md<T, 6>(in, out, nb, flags, band_active, vloc, nlbuf);
+    case 8: return launch_hpsi_md<T, 8>(in, out, nb, flags, band_active, vloc, nlbuf);
     default: return set_error(kErrArg, "Md=%d not compiled (1,2,3,4,6,8)", c.g.Md);
     }
 }
@@ -539,6 +592,39 @@ int nl_apply(T *out, int nb, const T *coef, const int *band_active)
     return kOK;
 }
 
+template <class T, int P>
+static void launch_nl_gather(int nb, const T *coef, const int *band_active, T *nlbuf)
+{
+    Context &c = ctx();
+    NonlocalTables &nl = c.nl;
+    constexpr int NBB = 16;
+    const int nblk = (nl.ncov + 255) / 256;
+    int ngroups = std::max(1, std::min((nb + NBB - 1) / NBB, (4096 + nblk - 1) / nblk));
+    int bpg = (nb + ngroups - 1) / ngroups;
+    bpg = (bpg + NBB - 1) / NBB * NBB;
+    ngroups = (nb + bpg - 1) / bpg;
+    const T *val = sizeof(T) == 8 ? (const T *)nl.g_val64.p : (const T *)nl.g_val32.p;
+    hipLaunchKernelGGL((k_nl_gather<T, P, NBB>), dim3(nblk, ngroups), dim3(256), 0, c.stream, nlbuf,
+                       nl.ncov, nb, bpg, nl.mem_ptr.as<int>(), nl.mem_pos.as<int>(),
+                       nl.mem_g.as<int>(), val, nl.g_lma.as<int>(), coef, band_active);
+}
+
+template <class T>
+static int nl_gather(int nb, const T *coef, const int *band_active, T *nlbuf)
+{
+    Context &c = ctx();
+    switch (c.nl.gp) {
+    case 1: launch_nl_gather<T, 1>(nb, coef, band_active, nlbuf); break;
+    case 2: launch_nl_gather<T, 2>(nb, coef, band_active, nlbuf); break;
+    case 4: launch_nl_gather<T, 4>(nb, coef, band_active, nlbuf); break;
+    case 8: launch_nl_gather<T, 8>(nb, coef, band_active, nlbuf); break;
+    case 16: launch_nl_gather<T, 16>(nb, coef, band_active, nlbuf); break;
+    default: return set_error(kErrState, "projector group width %d", c.nl.gp);
+    }
+    RS_HIP(hipGetLastError());
+    return kOK;
+}
+
 template <class T>
 int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
 {
@@ -549,6 +635,16 @@ int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
     if (flags & kNl) {
         RS_TRY(c.coefC.ensure((size_t)nb * c.nl.nz * sizeof(T)));
         RS_TRY(nl_project<T>(in, nb, c.coefC.as<T>(), band_active));
+    }
+    // FP32 H psi with kFuseNonlocal: the nonlocal term goes through the
+    // compact buffer into the stencil epilogue (one write of the output)
+    if (kFuseNonlocal && sizeof(T) == 4 && sflags == (kKin | kLoc) && (flags & kNl) &&
+        c.nl.ncov > 0) {
+        RS_TRY(c.nlbuf.ensure((size_t)nb * c.nl.ncov * sizeof(T)));
+        RS_TRY(nl_gather<T>(nb, c.coefC.as<T>(), band_active, c.nlbuf.as<T>()));
+        RS_TRY(launch_stencil<T>(in, out, nb, kKin | kLoc | kNl, band_active, c.nlbuf.as<T>()));
+        c.hpsi_bands += nb;
+        return kOK;
     }
     if (sflags & (kKin | kLoc)) {
         RS_TRY(launch_stencil<T>(in, out, nb, sflags, band_active));
@@ -563,7 +659,41 @@ int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
 template <class T>
 int hpsi_stencil_only(const T *in, T *out, int nb)
 {
+    Context &c = ctx();
+    if (kFuseNonlocal && sizeof(T) == 4 && c.nl.nz > 0 && c.nl.ncov > 0) {
+        RS_TRY(c.nlbuf.ensure((size_t)nb * c.nl.ncov * sizeof(T)));
+        return launch_stencil<T>(in, out, nb, kKin | kLoc | kNl, nullptr, c.nlbuf.as<T>());
+    }
     return launch_stencil<T>(in, out, nb, kKin | kLoc, nullptr);
+}
+
+template <class T>
+int nl_stage(T *out, int nb, const T *coef)
+{
+    Context &c = ctx();
+    if (kFuseNonlocal && sizeof(T) == 4 && c.nl.nz > 0 && c.nl.ncov > 0) {
+        RS_TRY(c.nlbuf.ensure((size_t)nb * c.nl.ncov * sizeof(T)));
+        return nl_gather<T>(nb, coef, nullptr, c.nlbuf.as<T>());
+    }
+    return nl_apply<T>(out, nb, coef, nullptr);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_copy_bands(const T *__restrict__ in, T *__restrict__ out, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+template <class T>
+int copy_bands(const T *in, T *out, int nb)
+{
+    Context &c = ctx();
+    const long n = c.g.ML * nb;
+    hipLaunchKernelGGL((k_copy_bands<T>), dim3(8192), dim3(256), 0, c.stream, in, out, n);
+    RS_HIP(hipGetLastError());
+    return kOK;
 }
 
 template <class T>
@@ -590,6 +720,10 @@ template int hpsi_apply<double>(const double *, double *, int, int, const int *)
 template int hpsi_apply<float>(const float *, float *, int, int, const int *);
 template int hpsi_stencil_only<double>(const double *, double *, int);
 template int hpsi_stencil_only<float>(const float *, float *, int);
+template int nl_stage<double>(double *, int, const double *);
+template int nl_stage<float>(float *, int, const float *);
+template int copy_bands<double>(const double *, double *, int);
+template int copy_bands<float>(const float *, float *, int);
 template int nl_project<double>(const double *, int, double *, const int *);
 template int nl_apply<double>(double *, int, const double *, const int *);
 template int nl_apply<float>(float *, int, const float *, const int *);

@@ -52,17 +52,26 @@ __host__ __device__ constexpr int qout(int l, int NLEV) {
     return qoff(l, NLEV) + qdepth(l, NLEV) - 1 - (NLEV - l);
 }
 
-// MODE 0: Gram pass, NLEV = m + 1, part[(j*(NB)+b)*tiles + tile] over the
-//         (m+1)(m+2)/2 entries j <= k of G (row-major upper triangle).
+// MODE 0: Gram pass, NLEV = m + 1, part[(e*nb+b)*tiles + tile] over the
+//         (m+1)(m+2)/2 entries j <= k of the Gram matrix of v_0 .. v_m
+//         (levels 1..NLEV, row-major upper triangle), then (g,g) and
+//         (g, v_k) for k < m (level 0 against levels 0..NLEV-1).
 // MODE 1: apply pass, NLEV = m, PG = g + sum_{k<m} d[b][k] v_k.
+// MODE 2: apply pass fused with the CG direction update (cg_module.f90:258-
+//         266): PG = g + sum_{k<m} d[b][k] v_k + beta[b] PG (beta == null:
+//         PG is overwritten, the first CG step).
+// FP64 keeps 768 threads so the compiler may use up to 168 registers
+template <class T>
+constexpr int sstep_max_threads() { return sizeof(T) == 8 ? 768 : 1024; }
+
 template <class T, int PX, int NLEV, int MODE>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(sstep_max_threads<T>())
 k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2, int N3,
            int ty, int tz, int tpr, double c0, double c1, double c2, double c3, double s1,
            int nb, const int *__restrict__ active, const double *__restrict__ dcoef,
-           double *__restrict__ part, int tiles)
+           double *__restrict__ part, int tiles, const double *__restrict__ beta)
 {
-    constexpr int NG = (NLEV) * (NLEV + 1) / 2;       // Gram entries (levels 1..NLEV)
+    constexpr int NG = NLEV * (NLEV + 1) / 2 + NLEV;  // Gram entries
     constexpr int QTOT = qoff(NLEV, NLEV) + 1;
     extern __shared__ unsigned char smem_raw[];
     T *Lbase = reinterpret_cast<T *>(smem_raw);
@@ -89,10 +98,11 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
     const bool interior = r >= NLEV && r < NLEV + ty;
 
     double dk[NLEV > 0 ? NLEV : 1];
-    if (MODE == 1) {
+    if (MODE >= 1) {
 #pragma unroll
         for (int k = 0; k < NLEV; ++k) dk[k] = dcoef[(long)band * NLEV + k];
     }
+    const double bk = (MODE == 2 && beta) ? beta[band] : 0.0;
     double gacc[MODE == 0 ? NG : 1];
 #pragma unroll
     for (int k = 0; k < (MODE == 0 ? NG : 1); ++k) gacc[k] = 0.0;
@@ -155,19 +165,22 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
         const int zo = s - NLEV;
         if (interior && zo >= z0) {
             if (MODE == 0) {
-                T vv[NLEV][PX];
+                T vv[NLEV + 1][PX];
 #pragma unroll
-                for (int l = 1; l <= NLEV; ++l)
+                for (int l = 0; l <= NLEV; ++l)
 #pragma unroll
-                    for (int px = 0; px < PX; ++px) vv[l - 1][px] = q[qout(l, NLEV)][px];
+                    for (int px = 0; px < PX; ++px) vv[l][px] = q[qout(l, NLEV)][px];
 #pragma unroll
                 for (int px = 0; px < PX; ++px) {
                     int e = 0;
 #pragma unroll
-                    for (int j = 0; j < NLEV; ++j)
+                    for (int j = 1; j <= NLEV; ++j)
 #pragma unroll
-                        for (int k = j; k < NLEV; ++k)
+                        for (int k = j; k <= NLEV; ++k)
                             gacc[e++] += (double)vv[j][px] * (double)vv[k][px];
+#pragma unroll
+                    for (int k = 0; k < NLEV; ++k)
+                        gacc[e++] += (double)vv[0][px] * (double)vv[k][px];
                 }
             } else {
                 T *dst = PG + (long)band * ML + (long)zo * plane + rowoff;
@@ -176,7 +189,14 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
                     double x = (double)q[qout(0, NLEV)][px];
 #pragma unroll
                     for (int k = 0; k < NLEV; ++k) x += dk[k] * (double)q[qout(k + 1, NLEV)][px];
-                    dst[px * tpr] = (T)x;
+                    if (MODE == 2) {
+                        // P = Pg + beta P in the working precision, as k_cg_newp
+                        T p = (T)x;
+                        if (beta) p = p + (T)bk * dst[px * tpr];
+                        dst[px * tpr] = p;
+                    } else {
+                        dst[px * tpr] = (T)x;
+                    }
                 }
             }
         }
@@ -209,14 +229,17 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
 // Per band: Gram matrix -> the reference's mloop CG steps in coefficient
 // space -> d[b][0..m-1] (Pg = g + sum d_k v_k); pc_active[b] = 0 when
 // (r0,r0) < 1e-24 (the reference's early return), which gives d = 0.
+// pgg[b] = (Pg, g) = (g,g) + sum_k d_k (v_k, g) from the same Gram matrix
+// (the inner product the CG step takes next, cg_module.f90:237-243).
 template <int M>
 __global__ void k_pc_sstep_scalars(const double *__restrict__ part, int nb, int tiles, double lmax,
-                                   const int *__restrict__ active, double *__restrict__ dcoef)
+                                   const int *__restrict__ active, double *__restrict__ dcoef,
+                                   double *__restrict__ pgg)
 {
     constexpr int N = M + 1;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb || !active[b]) return;
-    double Gm[N][N];
+    double Gm[N][N], Gg[N];
     int e = 0;
     for (int j = 0; j < N; ++j)
         for (int k = j; k < N; ++k) {
@@ -227,6 +250,13 @@ __global__ void k_pc_sstep_scalars(const double *__restrict__ part, int nb, int 
             Gm[k][j] = t;
             ++e;
         }
+    for (int k = 0; k < N; ++k) {   // Gg[0] = (g,g), Gg[k+1] = (g, v_k)
+        double t = 0.0;
+        const double *p = part + ((long)e * nb + b) * tiles;
+        for (int c = 0; c < tiles; ++c) t += p[c];
+        Gg[k] = t;
+        ++e;
+    }
     double d[M];
     for (int k = 0; k < M; ++k) d[k] = 0.0;
     double rr1 = Gm[0][0];
@@ -263,72 +293,108 @@ __global__ void k_pc_sstep_scalars(const double *__restrict__ part, int nb, int 
             for (int k = 0; k < N; ++k) pc[k] = rc[k] + bb * pc[k];
         }
     }
-    for (int k = 0; k < M; ++k) dcoef[(long)b * M + k] = d[k];
+    double pg = Gg[0];
+    for (int k = 0; k < M; ++k) {
+        dcoef[(long)b * M + k] = d[k];
+        pg += d[k] * Gg[k + 1];
+    }
+    if (pgg) pgg[b] = pg;
 }
 
 struct SstepGeom {
     int ty, tz, tiles, threads, px;
 };
 
-static SstepGeom sstep_geom(const GridDesc &g, int nlev, int px)
+static SstepGeom sstep_geom(const GridDesc &g, int nlev, int px, int maxthr)
 {
     SstepGeom G{};
     G.px = px;
     const int tpr = g.N1 / px;
+    // tallest tile that fits: every level recomputes one halo row per side
     G.ty = 0;
     for (int ty = 1; ty <= g.N2; ++ty) {
         if (g.N2 % ty) continue;
         const int thr = tpr * (ty + 2 * nlev);
-        if (thr > 1024) break;
-        if (ty >= 8 || ty == g.N2) { G.ty = ty; G.threads = thr; if (thr >= 384) break; }
+        if (thr > maxthr) break;
+        G.ty = ty;
+        G.threads = thr;
     }
-    if (G.ty == 0) {
-        for (int ty = g.N2; ty >= 1; --ty)
-            if (g.N2 % ty == 0 && tpr * (ty + 2 * nlev) <= 1024) { G.ty = ty; G.threads = tpr * (ty + 2 * nlev); break; }
-    }
+    // the whole z column when short (the 2*nlev warm-up planes are paid once)
     int tz = g.N3;
-    for (int c = 1; c <= g.N3; ++c)
-        if (g.N3 % c == 0 && g.N3 / c <= 24) { tz = g.N3 / c; break; }
+    if (g.N3 > 96)
+        for (int d = 96; d >= 1; --d)
+            if (g.N3 % d == 0) { tz = d; break; }
     G.tz = tz;
-    G.tiles = (g.N2 / G.ty) * (g.N3 / G.tz);
+    G.tiles = G.ty ? (g.N2 / G.ty) * (g.N3 / G.tz) : 0;
     return G;
 }
 
+struct SstepConst {
+    double c0, c1, c2, c3, lmax, s1;
+};
+
+static SstepConst sstep_const(const GridDesc &gd)
+{
+    SstepConst k;
+    const double hx = 1.0 / (gd.h[0] * gd.h[0]), hy = 1.0 / (gd.h[1] * gd.h[1]),
+                 hz = 1.0 / (gd.h[2] * gd.h[2]);
+    k.c0 = hx + hy + hz;
+    k.c1 = -0.5 * hx;
+    k.c2 = -0.5 * hy;
+    k.c3 = -0.5 * hz;
+    k.lmax = 2.0 * k.c0;
+    k.s1 = 2.0 / k.lmax;
+    return k;
+}
+
+// pass 1: Gram of g, v_0 .. v_M, then the coefficients d and (Pg, g)
 template <class T, int M>
-static int sstep_launch(const T *g, T *pg, int nb, const int *active, double *part, double *dcoef)
+static int sstep_coeffs(const T *g, int nb, const int *active, double *part, double *dcoef,
+                        double *pgg)
 {
     Context &c = ctx();
     const GridDesc &gd = c.g;
-    constexpr int PX = sizeof(T) == 8 ? 2 : 4;
-    const double hx = 1.0 / (gd.h[0] * gd.h[0]), hy = 1.0 / (gd.h[1] * gd.h[1]),
-                 hz = 1.0 / (gd.h[2] * gd.h[2]);
-    const double c0 = hx + hy + hz, c1 = -0.5 * hx, c2 = -0.5 * hy, c3 = -0.5 * hz;
-    const double lmax = 2.0 * c0;
-    const double s1 = 2.0 / lmax;
-    // pass 1: Gram of v_0 .. v_M
-    {
-        const SstepGeom G = sstep_geom(gd, M + 1, PX);
-        if (G.ty == 0) return set_error(kErrArg, "no s-step tiling");
-        const size_t lds = (size_t)(M + 1) * (G.ty + 2 * (M + 1)) * (gd.N1 + 2) * sizeof(T);
-        const long nblk = (long)G.tiles * nb;
-        hipLaunchKernelGGL((k_pc_sstep<T, PX, M + 1, 0>), dim3((nblk + 7) / 8 * 8), dim3(G.threads),
-                           lds, c.stream, g, pg, gd.ML, gd.N1, gd.N2, gd.N3, G.ty, G.tz, gd.N1 / PX,
-                           c0, c1, c2, c3, s1, nb, active, (const double *)nullptr, part, G.tiles);
-        RS_HIP(hipGetLastError());
-        hipLaunchKernelGGL((k_pc_sstep_scalars<M>), dim3((nb + 127) / 128), dim3(128), 0, c.stream,
-                           part, nb, G.tiles, lmax, active, dcoef);
-        RS_HIP(hipGetLastError());
-    }
-    // pass 2: Pg = g + sum d_k v_k
-    {
-        const SstepGeom G = sstep_geom(gd, M, PX);
-        const size_t lds = (size_t)M * (G.ty + 2 * M) * (gd.N1 + 2) * sizeof(T);
-        const long nblk = (long)G.tiles * nb;
+    constexpr int PX = 2;
+    const SstepConst k = sstep_const(gd);
+    const SstepGeom G = sstep_geom(gd, M + 1, PX, sstep_max_threads<T>());
+    if (G.ty == 0) return set_error(kErrArg, "no s-step tiling");
+    const size_t lds = (size_t)(M + 1) * (G.ty + 2 * (M + 1)) * (gd.N1 + 2) * sizeof(T);
+    const long nblk = (long)G.tiles * nb;
+    hipLaunchKernelGGL((k_pc_sstep<T, PX, M + 1, 0>), dim3((nblk + 7) / 8 * 8), dim3(G.threads),
+                       lds, c.stream, g, (T *)nullptr, gd.ML, gd.N1, gd.N2, gd.N3, G.ty, G.tz,
+                       gd.N1 / PX, k.c0, k.c1, k.c2, k.c3, k.s1, nb, active,
+                       (const double *)nullptr, part, G.tiles, (const double *)nullptr);
+    RS_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_pc_sstep_scalars<M>), dim3((nb + 127) / 128), dim3(128), 0, c.stream,
+                       part, nb, G.tiles, k.lmax, active, dcoef, pgg);
+    RS_HIP(hipGetLastError());
+    return kOK;
+}
+
+// pass 2: out = g + sum d_k v_k (fuse == 0), or out = that + beta out (fuse == 1)
+template <class T, int M>
+static int sstep_apply(const T *g, T *out, int nb, const int *active, const double *dcoef,
+                       int fuse, const double *beta)
+{
+    Context &c = ctx();
+    const GridDesc &gd = c.g;
+    constexpr int PX = 2;
+    const SstepConst k = sstep_const(gd);
+    const SstepGeom G = sstep_geom(gd, M, PX, sstep_max_threads<T>());
+    if (G.ty == 0) return set_error(kErrArg, "no s-step tiling");
+    const size_t lds = (size_t)M * (G.ty + 2 * M) * (gd.N1 + 2) * sizeof(T);
+    const long nblk = (long)G.tiles * nb;
+    if (fuse)
+        hipLaunchKernelGGL((k_pc_sstep<T, PX, M, 2>), dim3((nblk + 7) / 8 * 8), dim3(G.threads),
+                           lds, c.stream, g, out, gd.ML, gd.N1, gd.N2, gd.N3, G.ty, G.tz,
+                           gd.N1 / PX, k.c0, k.c1, k.c2, k.c3, k.s1, nb, active, dcoef,
+                           (double *)nullptr, G.tiles, beta);
+    else
         hipLaunchKernelGGL((k_pc_sstep<T, PX, M, 1>), dim3((nblk + 7) / 8 * 8), dim3(G.threads),
-                           lds, c.stream, g, pg, gd.ML, gd.N1, gd.N2, gd.N3, G.ty, G.tz, gd.N1 / PX,
-                           c0, c1, c2, c3, s1, nb, active, (const double *)dcoef, part, G.tiles);
-        RS_HIP(hipGetLastError());
-    }
+                           lds, c.stream, g, out, gd.ML, gd.N1, gd.N2, gd.N3, G.ty, G.tz,
+                           gd.N1 / PX, k.c0, k.c1, k.c2, k.c3, k.s1, nb, active, dcoef,
+                           (double *)nullptr, G.tiles, (const double *)nullptr);
+    RS_HIP(hipGetLastError());
     return kOK;
 }
 
@@ -337,10 +403,10 @@ size_t precond_sstep_part_elems(int nb)
     Context &c = ctx();
     size_t mx = 0;
     for (int m = 1; m <= 4; ++m) {
-        const SstepGeom G = sstep_geom(c.g, m + 1, 2);
-        const SstepGeom G4 = sstep_geom(c.g, m + 1, 4);
+        const SstepGeom G = sstep_geom(c.g, m + 1, 2, 768);
+        const SstepGeom G4 = sstep_geom(c.g, m + 1, 2, 1024);
         const size_t t = (size_t)std::max(G.tiles, G4.tiles);
-        mx = std::max(mx, t * (size_t)((m + 1) * (m + 2) / 2) * nb);
+        mx = std::max(mx, t * (size_t)((m + 1) * (m + 2) / 2 + m + 1) * nb);
     }
     return mx + (size_t)4 * nb;
 }
@@ -348,16 +414,45 @@ size_t precond_sstep_part_elems(int nb)
 template <class T>
 int precond_sstep(const T *g, T *pg, int nb, const int *active, double *part, double *dcoef)
 {
+    RS_TRY(precond_sstep_coeffs<T>(g, nb, active, part, dcoef, nullptr));
+    return precond_sstep_apply<T>(g, pg, nb, active, dcoef, 0, nullptr);
+}
+
+template <class T>
+int precond_sstep_coeffs(const T *g, int nb, const int *active, double *part, double *dcoef,
+                         double *pgg)
+{
     switch (ctx().mloop) {
-    case 1: return sstep_launch<T, 1>(g, pg, nb, active, part, dcoef);
-    case 2: return sstep_launch<T, 2>(g, pg, nb, active, part, dcoef);
-    case 3: return sstep_launch<T, 3>(g, pg, nb, active, part, dcoef);
-    case 4: return sstep_launch<T, 4>(g, pg, nb, active, part, dcoef);
+    case 1: return sstep_coeffs<T, 1>(g, nb, active, part, dcoef, pgg);
+    case 2: return sstep_coeffs<T, 2>(g, nb, active, part, dcoef, pgg);
+    case 3: return sstep_coeffs<T, 3>(g, nb, active, part, dcoef, pgg);
+    case 4: return sstep_coeffs<T, 4>(g, nb, active, part, dcoef, pgg);
+    default: return set_error(kErrArg, "s-step preconditioner built for mloop 1..4");
+    }
+}
+
+template <class T>
+int precond_sstep_apply(const T *g, T *out, int nb, const int *active, const double *dcoef,
+                        int fuse, const double *beta)
+{
+    switch (ctx().mloop) {
+    case 1: return sstep_apply<T, 1>(g, out, nb, active, dcoef, fuse, beta);
+    case 2: return sstep_apply<T, 2>(g, out, nb, active, dcoef, fuse, beta);
+    case 3: return sstep_apply<T, 3>(g, out, nb, active, dcoef, fuse, beta);
+    case 4: return sstep_apply<T, 4>(g, out, nb, active, dcoef, fuse, beta);
     default: return set_error(kErrArg, "s-step preconditioner built for mloop 1..4");
     }
 }
 
 template int precond_sstep<double>(const double *, double *, int, const int *, double *, double *);
 template int precond_sstep<float>(const float *, float *, int, const int *, double *, double *);
+template int precond_sstep_coeffs<double>(const double *, int, const int *, double *, double *,
+                                          double *);
+template int precond_sstep_coeffs<float>(const float *, int, const int *, double *, double *,
+                                         double *);
+template int precond_sstep_apply<double>(const double *, double *, int, const int *,
+                                         const double *, int, const double *);
+template int precond_sstep_apply<float>(const float *, float *, int, const int *, const double *,
+                                        int, const double *);
 
 }  // namespace rsdft

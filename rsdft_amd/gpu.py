@@ -265,8 +265,9 @@ class RsdftGpu:
     # -- measurement ---------------------------------------------------------------------
     def test_hpsi(self, nrep: int = 10):
         """ms per all-band application: hamiltonian, kinetic, localpot, nonlocal,
-        stencil kernel, projection kernel, nonlocal apply kernel."""
-        t = np.zeros(7)
+        stencil kernel of the hamiltonian, projection kernel, nonlocal term
+        stage, plain copy of all bands."""
+        t = np.zeros(8)
         self._call("rsdft_gpu_test_hpsi", nrep, _p(t))
         return t
 
