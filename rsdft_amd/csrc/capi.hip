@@ -208,6 +208,10 @@ int rsdft_gpu_init(int N1, int N2, int N3, const double *Hgrid, double dV, int M
         return set_error(kErrArg, "grid smaller than the stencil");
     if (!c.init) {
         RS_HIP(hipGetDevice(&c.device));
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device) ==
+                hipSuccess && lds > 0)
+            c.max_lds = (size_t)lds;
         RS_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         RS_BLAS(rocblas_create_handle(&c.blas));
         RS_BLAS(rocblas_set_stream(c.blas, c.stream));

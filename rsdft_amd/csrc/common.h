@@ -154,6 +154,7 @@ struct Context {
     int prec = 0;
     int ncg = 3, mloop = 3;
     int pc_mode = 1;   // 0: step-by-step preconditioner recurrence, 1: s-step form
+    size_t max_lds = 64 * 1024;   // LDS bytes one workgroup may allocate on this device
     int eig_mode = 1;  // 0: always dsyevd, 1: refinement when Hsub is nearly diagonal
     long eig_refined = 0;
 

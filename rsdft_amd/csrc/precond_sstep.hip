@@ -40,16 +40,22 @@ __device__ __forceinline__ long xcd_map(long L, long total) {
 }
 
 // Queue depth of level l for NLEV levels above g: three planes for the next
-// level's stencil, and back to plane s - NLEV where all levels are combined.
+// level's stencil, and back to plane s - NLEV where all levels are combined;
+// level 0 (g) also holds the PF planes requested ahead.
+constexpr int kSstepPF = 2;
 __host__ __device__ constexpr int qdepth(int l, int NLEV) {
-    return l == NLEV ? 1 : ((NLEV - l + 1) > 3 ? (NLEV - l + 1) : 3);
+    return l == NLEV ? 1 : ((NLEV - l + 1) > 3 ? (NLEV - l + 1) : 3) + (l == 0 ? kSstepPF : 0);
 }
 __host__ __device__ constexpr int qoff(int l, int NLEV) {
     return l == 0 ? 0 : qoff(l - 1, NLEV) + qdepth(l - 1, NLEV);
 }
+// slot of the newest computed plane of level l (level 0: the current plane s)
+__host__ __device__ constexpr int qnew(int l, int NLEV) {
+    return qoff(l, NLEV) + qdepth(l, NLEV) - 1 - (l == 0 ? kSstepPF : 0);
+}
 // slot of level l holding plane s - NLEV (the plane where all levels meet)
 __host__ __device__ constexpr int qout(int l, int NLEV) {
-    return qoff(l, NLEV) + qdepth(l, NLEV) - 1 - (NLEV - l);
+    return qnew(l, NLEV) - (NLEV - l);
 }
 
 // MODE 0: Gram pass, NLEV = m + 1, part[(e*nb+b)*tiles + tile] over the
@@ -60,6 +66,13 @@ __host__ __device__ constexpr int qout(int l, int NLEV) {
 // MODE 2: apply pass fused with the CG direction update (cg_module.f90:258-
 //         266): PG = g + sum_{k<m} d[b][k] v_k + beta[b] PG (beta == null:
 //         PG is overwritten, the first CG step).
+//
+// Step s: level 0's current plane is s (requested kSstepPF steps earlier);
+// level l = 1..NLEV computes plane s - l from level l-1 at planes s-l-1,
+// s-l, s-l+1 (registers; s-l+1 was computed earlier in this step) with the
+// x/y neighbours of plane s-l taken from LDS, where level l-1 left it at the
+// end of step s-1.  Every level's newest plane is staged into one of two LDS
+// buffers at the end of the step, so one barrier per plane suffices.
 // FP64 keeps 768 threads so the compiler may use up to 168 registers
 template <class T>
 constexpr int sstep_max_threads() { return sizeof(T) == 8 ? 768 : 1024; }
@@ -73,6 +86,7 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
 {
     constexpr int NG = NLEV * (NLEV + 1) / 2 + NLEV;  // Gram entries
     constexpr int QTOT = qoff(NLEV, NLEV) + 1;
+    constexpr int PF = kSstepPF;
     extern __shared__ unsigned char smem_raw[];
     T *Lbase = reinterpret_cast<T *>(smem_raw);
     __shared__ double red[16][MODE == 0 ? NG : 1];
@@ -88,6 +102,7 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
     const int z0 = (tile / nty) * tz;
     const int rows = ty + 2 * NLEV;
     const int W = N1 + 2;
+    const int LS = rows * W;              // one level plane in LDS
     const int tid = threadIdx.x;
     const int r = tid / tpr;                 // 0 .. rows-1
     const int cx0 = tid - r * tpr;
@@ -107,107 +122,140 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
 #pragma unroll
     for (int k = 0; k < (MODE == 0 ? NG : 1); ++k) gacc[k] = 0.0;
 
+    // levels 1..NLEV: register queues shifted once per step (computed values);
+    // level 0 (g, loaded from HBM) is a ring indexed by the unrolled step, so
+    // a request stays in flight for PF steps without a register move
     T q[QTOT][PX];
 #pragma unroll
     for (int k = 0; k < QTOT; ++k)
 #pragma unroll
         for (int px = 0; px < PX; ++px) q[k][px] = T(0);
+    constexpr int D0 = qdepth(0, NLEV);
+    T g0[D0][PX];
+    const int s0 = z0 - NLEV, s1e = z0 + tz + NLEV;
+    // ring slot of plane p: (p - P0) mod D0, P0 = s0 - (D0 - 1 - PF)
+#pragma unroll
+    for (int k = 0; k < D0 - 1; ++k) {
+        const int zz = s0 - (D0 - 1 - PF) + k;
+        const T *p = src + (long)wrapz(zz, N3) * plane + rowoff;
+#pragma unroll
+        for (int px = 0; px < PX; ++px) g0[k][px] = p[px * tpr];
+    }
 
     const T tc0 = (T)c0, tc1 = (T)c1, tc2 = (T)c2, tc3 = (T)c3, ts1 = (T)s1;
-    for (int s = z0 - NLEV; s < z0 + tz + NLEV; ++s) {
-        // level 0: newest plane s of g
-        {
-            const T *p = src + (long)wrapz(s, N3) * plane + rowoff;
+    for (int sb = s0; sb < s1e; sb += D0) {
 #pragma unroll
-            for (int px = 0; px < PX; ++px) q[qoff(0, NLEV) + qdepth(0, NLEV) - 1][px] = p[px * tpr];
-        }
+        for (int u = 0; u < D0; ++u) {
+            const int s = sb + u;
+            if (s >= s1e) break;
+            // slot of level-0 plane s - k at this step
+#define G0(k) g0[(u + 2 * D0 - 1 - PF - (k)) % D0]
+            T *Lcur = Lbase + (long)((s - s0) & 1) * NLEV * LS;       // written at the end of this step
+            const T *Lprev = Lbase + (long)(((s - s0) & 1) ^ 1) * NLEV * LS;   // left by step s-1
+            {   // request plane s + PF of g
+                int zn = s + PF;
+                while (zn < 0) zn += N3;
+                while (zn >= N3) zn -= N3;
+                const T *p = src + (long)zn * plane + rowoff;
 #pragma unroll
-        for (int l = 0; l < NLEV; ++l) {
-            // stage level l at plane s-l-1 (queue slot depth-2) into LDS plane l
-            T *Lp = Lbase + (long)l * rows * W;
-            const int qs = qoff(l, NLEV) + qdepth(l, NLEV) - 2;
-            if (r >= l && r < rows - l) {
+                for (int px = 0; px < PX; ++px) G0(-PF)[px] = p[px * tpr];
+            }
 #pragma unroll
-                for (int px = 0; px < PX; ++px) {
-                    const int x = cx0 + px * tpr;
-                    Lp[r * W + 1 + x] = q[qs][px];
-                    if (x == 0) Lp[r * W + 1 + N1] = q[qs][px];
-                    if (x == N1 - 1) Lp[r * W] = q[qs][px];
+            for (int l = 0; l < NLEV; ++l) {
+                // level l+1 at plane s-l-1 on rows [l+1, rows-l-1): level l at planes
+                // s-l-2 .. s-l, x/y of plane s-l-1 from Lprev
+                const T *Lp = Lprev + (long)l * LS;
+                const int qn = qnew(l + 1, NLEV);
+                if (r >= l + 1 && r < rows - l - 1) {
+#pragma unroll
+                    for (int px = 0; px < PX; ++px) {
+                        const int x = cx0 + px * tpr;
+                        const T *lr = Lp + r * W + 1 + x;
+                        const T cen = l == 0 ? G0(1)[px] : q[qnew(l, NLEV) - 1][px];
+                        const T zp = l == 0 ? G0(0)[px] : q[qnew(l, NLEV)][px];
+                        const T zm = l == 0 ? G0(2)[px] : q[qnew(l, NLEV) - 2][px];
+                        T a = tc0 * cen;
+                        a = a + tc1 * (lr[1] + lr[-1]) + tc2 * (lr[W] + lr[-W]) + tc3 * (zp + zm);
+                        T v;
+                        if (l == 0) {
+                            v = cen - a;                          // v_0 = g - A g
+                        } else if (l == 1) {
+                            v = ts1 * a - cen;                    // v_1 = tau(A) v_0
+                        } else {
+                            // v_l = 2 tau(A) v_{l-1} - v_{l-2}; v_{l-2} is level l-1 at this plane
+                            const T prev = q[qnew(l - 1, NLEV) - 2][px];
+                            v = T(2) * (ts1 * a - cen) - prev;
+                        }
+                        q[qn][px] = v;
+                    }
                 }
             }
+            // all levels at plane zo = s - NLEV
+            const int zo = s - NLEV;
+            if (interior && zo >= z0) {
+                if (MODE == 0) {
+                    T vv[NLEV + 1][PX];
+#pragma unroll
+                    for (int px = 0; px < PX; ++px) vv[0][px] = G0(NLEV)[px];
+#pragma unroll
+                    for (int l = 1; l <= NLEV; ++l)
+#pragma unroll
+                        for (int px = 0; px < PX; ++px) vv[l][px] = q[qout(l, NLEV)][px];
+#pragma unroll
+                    for (int px = 0; px < PX; ++px) {
+                        int e = 0;
+#pragma unroll
+                        for (int j = 1; j <= NLEV; ++j)
+#pragma unroll
+                            for (int k = j; k <= NLEV; ++k)
+                                gacc[e++] += (double)vv[j][px] * (double)vv[k][px];
+#pragma unroll
+                        for (int k = 0; k < NLEV; ++k)
+                            gacc[e++] += (double)vv[0][px] * (double)vv[k][px];
+                    }
+                } else {
+                    T *dst = PG + (long)band * ML + (long)zo * plane + rowoff;
+#pragma unroll
+                    for (int px = 0; px < PX; ++px) {
+                        double x = (double)G0(NLEV)[px];
+#pragma unroll
+                        for (int k = 0; k < NLEV; ++k) x += dk[k] * (double)q[qout(k + 1, NLEV)][px];
+                        if (MODE == 2) {
+                            // P = Pg + beta P in the working precision, as k_cg_newp
+                            T p = (T)x;
+                            if (beta) p = p + (T)bk * dst[px * tpr];
+                            dst[px * tpr] = p;
+                        } else {
+                            dst[px * tpr] = (T)x;
+                        }
+                    }
+                }
+            }
+            // stage the newest plane of levels 0 .. NLEV-1 for the next step
+#pragma unroll
+            for (int l = 0; l < NLEV; ++l) {
+                if (r >= l && r < rows - l) {
+                    T *Lp = Lcur + (long)l * LS;
+#pragma unroll
+                    for (int px = 0; px < PX; ++px) {
+                        const int x = cx0 + px * tpr;
+                        const T v = l == 0 ? G0(0)[px] : q[qnew(l, NLEV)][px];
+                        Lp[r * W + 1 + x] = v;
+                        if (x == 0) Lp[r * W + 1 + N1] = v;
+                        if (x == N1 - 1) Lp[r * W] = v;
+                    }
+                }
+            }
+#undef G0
             __syncthreads();
-            // level l+1 at plane s-l-1 on rows [l+1, rows-l-1)
-            const int qn = qoff(l + 1, NLEV) + qdepth(l + 1, NLEV) - 1;   // newest slot of l+1
-            if (r >= l + 1 && r < rows - l - 1) {
+            // shift the queues of levels 1..NLEV by one plane
 #pragma unroll
-                for (int px = 0; px < PX; ++px) {
-                    const int x = cx0 + px * tpr;
-                    const T *lr = Lp + r * W + 1 + x;
-                    const T cen = q[qs][px];
-                    T a = tc0 * cen;
-                    a = a + tc1 * (lr[1] + lr[-1]) + tc2 * (lr[W] + lr[-W]) +
-                        tc3 * (q[qs + 1][px] + q[qs - 1][px]);
-                    T v;
-                    if (l == 0) {
-                        v = cen - a;                          // v_0 = g - A g
-                    } else if (l == 1) {
-                        v = ts1 * a - cen;                    // v_1 = tau(A) v_0
-                    } else {
-                        // v_l = 2 tau(A) v_{l-1} - v_{l-2}; v_{l-2} is level l-1 at this plane
-                        const T prev = q[qoff(l - 1, NLEV) + qdepth(l - 1, NLEV) - 3][px];
-                        v = T(2) * (ts1 * a - cen) - prev;
-                    }
-                    q[qn][px] = v;
-                }
-            }
+            for (int l = 1; l <= NLEV; ++l)
+#pragma unroll
+                for (int k = 0; k < qdepth(l, NLEV) - 1; ++k)
+#pragma unroll
+                    for (int px = 0; px < PX; ++px) q[qoff(l, NLEV) + k][px] = q[qoff(l, NLEV) + k + 1][px];
         }
-        // all levels at plane zo = s - NLEV
-        const int zo = s - NLEV;
-        if (interior && zo >= z0) {
-            if (MODE == 0) {
-                T vv[NLEV + 1][PX];
-#pragma unroll
-                for (int l = 0; l <= NLEV; ++l)
-#pragma unroll
-                    for (int px = 0; px < PX; ++px) vv[l][px] = q[qout(l, NLEV)][px];
-#pragma unroll
-                for (int px = 0; px < PX; ++px) {
-                    int e = 0;
-#pragma unroll
-                    for (int j = 1; j <= NLEV; ++j)
-#pragma unroll
-                        for (int k = j; k <= NLEV; ++k)
-                            gacc[e++] += (double)vv[j][px] * (double)vv[k][px];
-#pragma unroll
-                    for (int k = 0; k < NLEV; ++k)
-                        gacc[e++] += (double)vv[0][px] * (double)vv[k][px];
-                }
-            } else {
-                T *dst = PG + (long)band * ML + (long)zo * plane + rowoff;
-#pragma unroll
-                for (int px = 0; px < PX; ++px) {
-                    double x = (double)q[qout(0, NLEV)][px];
-#pragma unroll
-                    for (int k = 0; k < NLEV; ++k) x += dk[k] * (double)q[qout(k + 1, NLEV)][px];
-                    if (MODE == 2) {
-                        // P = Pg + beta P in the working precision, as k_cg_newp
-                        T p = (T)x;
-                        if (beta) p = p + (T)bk * dst[px * tpr];
-                        dst[px * tpr] = p;
-                    } else {
-                        dst[px * tpr] = (T)x;
-                    }
-                }
-            }
-        }
-        // shift every queue by one plane
-#pragma unroll
-        for (int l = 0; l <= NLEV; ++l)
-#pragma unroll
-            for (int k = 0; k < qdepth(l, NLEV) - 1; ++k)
-#pragma unroll
-                for (int px = 0; px < PX; ++px) q[qoff(l, NLEV) + k][px] = q[qoff(l, NLEV) + k + 1][px];
-        if (NLEV == 1) __syncthreads();
     }
     if (MODE == 0) {
         const int lane = tid & 63, wave = tid >> 6, nw = (blockDim.x + 63) >> 6;
@@ -303,21 +351,27 @@ __global__ void k_pc_sstep_scalars(const double *__restrict__ part, int nb, int 
 
 struct SstepGeom {
     int ty, tz, tiles, threads, px;
+    size_t lds;
 };
 
-static SstepGeom sstep_geom(const GridDesc &g, int nlev, int px, int maxthr)
+// nlds: levels staged in LDS (two buffers each); the tallest tile that fits
+// the thread and LDS limits (every level recomputes one halo row per side)
+static SstepGeom sstep_geom(const GridDesc &g, int nlev, int nlds, int px, int maxthr,
+                            size_t elem, size_t maxlds)
 {
     SstepGeom G{};
     G.px = px;
     const int tpr = g.N1 / px;
-    // tallest tile that fits: every level recomputes one halo row per side
     G.ty = 0;
     for (int ty = 1; ty <= g.N2; ++ty) {
         if (g.N2 % ty) continue;
         const int thr = tpr * (ty + 2 * nlev);
         if (thr > maxthr) break;
+        const size_t lds = 2 * (size_t)nlds * (ty + 2 * nlev) * (g.N1 + 2) * elem;
+        if (lds > maxlds) break;
         G.ty = ty;
         G.threads = thr;
+        G.lds = lds;
     }
     // the whole z column when short (the 2*nlev warm-up planes are paid once)
     int tz = g.N3;
@@ -327,6 +381,15 @@ static SstepGeom sstep_geom(const GridDesc &g, int nlev, int px, int maxthr)
     G.tz = tz;
     G.tiles = G.ty ? (g.N2 / G.ty) * (g.N3 / G.tz) : 0;
     return G;
+}
+
+template <class K>
+static int allow_lds(K kernel, size_t lds)
+{
+    if (lds > 64 * 1024)
+        RS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    return kOK;
 }
 
 struct SstepConst {
@@ -356,10 +419,12 @@ static int sstep_coeffs(const T *g, int nb, const int *active, double *part, dou
     const GridDesc &gd = c.g;
     constexpr int PX = 2;
     const SstepConst k = sstep_const(gd);
-    const SstepGeom G = sstep_geom(gd, M + 1, PX, sstep_max_threads<T>());
+    const SstepGeom G = sstep_geom(gd, M + 1, M + 1, PX, sstep_max_threads<T>(), sizeof(T),
+                                   c.max_lds);
     if (G.ty == 0) return set_error(kErrArg, "no s-step tiling");
-    const size_t lds = (size_t)(M + 1) * (G.ty + 2 * (M + 1)) * (gd.N1 + 2) * sizeof(T);
+    const size_t lds = G.lds;
     const long nblk = (long)G.tiles * nb;
+    RS_TRY(allow_lds(k_pc_sstep<T, PX, M + 1, 0>, lds));
     hipLaunchKernelGGL((k_pc_sstep<T, PX, M + 1, 0>), dim3((nblk + 7) / 8 * 8), dim3(G.threads),
                        lds, c.stream, g, (T *)nullptr, gd.ML, gd.N1, gd.N2, gd.N3, G.ty, G.tz,
                        gd.N1 / PX, k.c0, k.c1, k.c2, k.c3, k.s1, nb, active,
@@ -380,10 +445,12 @@ static int sstep_apply(const T *g, T *out, int nb, const int *active, const doub
     const GridDesc &gd = c.g;
     constexpr int PX = 2;
     const SstepConst k = sstep_const(gd);
-    const SstepGeom G = sstep_geom(gd, M, PX, sstep_max_threads<T>());
+    const SstepGeom G = sstep_geom(gd, M, M, PX, sstep_max_threads<T>(), sizeof(T), c.max_lds);
     if (G.ty == 0) return set_error(kErrArg, "no s-step tiling");
-    const size_t lds = (size_t)M * (G.ty + 2 * M) * (gd.N1 + 2) * sizeof(T);
+    const size_t lds = G.lds;
     const long nblk = (long)G.tiles * nb;
+    RS_TRY(allow_lds(k_pc_sstep<T, PX, M, 1>, lds));
+    RS_TRY(allow_lds(k_pc_sstep<T, PX, M, 2>, lds));
     if (fuse)
         hipLaunchKernelGGL((k_pc_sstep<T, PX, M, 2>), dim3((nblk + 7) / 8 * 8), dim3(G.threads),
                            lds, c.stream, g, out, gd.ML, gd.N1, gd.N2, gd.N3, G.ty, G.tz,
@@ -403,8 +470,8 @@ size_t precond_sstep_part_elems(int nb)
     Context &c = ctx();
     size_t mx = 0;
     for (int m = 1; m <= 4; ++m) {
-        const SstepGeom G = sstep_geom(c.g, m + 1, 2, 768);
-        const SstepGeom G4 = sstep_geom(c.g, m + 1, 2, 1024);
+        const SstepGeom G = sstep_geom(c.g, m + 1, m + 1, 2, 768, 8, c.max_lds);
+        const SstepGeom G4 = sstep_geom(c.g, m + 1, m + 1, 2, 1024, 4, c.max_lds);
         const size_t t = (size_t)std::max(G.tiles, G4.tiles);
         mx = std::max(mx, t * (size_t)((m + 1) * (m + 2) / 2 + m + 1) * nb);
     }
