@@ -5,7 +5,9 @@
 //        the grid (373,248 points at Si216) and the output is small: split-K
 //        over workgroups, each writing an FP64 partial tile, then a
 //        fixed-order FP64 reduction (deterministic).
-//   NN:  Out(M x nb) = A(M x nb) * U(nb x nb)   (rotation Psi V, Psi R^-1)
+//   NN:  Out(M x nb) = A(M x nb) * U(nb x nb)   (rotation Psi V, Psi R^-1);
+//        mode 1 is a plain FP32 GEMM with a short K and goes to rocBLAS
+//        sgemm; mode 2 uses the 64 x 96 tile kernel below
 //
 // Two arithmetic modes on FP32 inputs:
 //   mode 1: v_mfma_f32_16x16x4_f32 products, FP32 accumulation;
@@ -361,6 +363,13 @@ k_gemm_nn(const float *__restrict__ A, const float *__restrict__ U, long M, int 
     }
 }
 
+__global__ void k_to_f32(const double *src, float *dst, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
 // ---------------------------------------------------------------------------
 // host wrappers
 // ---------------------------------------------------------------------------
@@ -390,10 +399,27 @@ static int gemm_tn_mode(const float *A, const float *B, int nb, long K, double a
     return kOK;
 }
 
+// mode 1: the rotation is a plain FP32 GEMM (K = nb is short), rocBLAS sgemm
+static int gemm_nn_sgemm(const float *A, const double *U, int nb, long M, float *Out)
+{
+    Context &c = ctx();
+    RS_TRY(c.matT.ensure(sizeof(float) * ((size_t)nb * nb + nb)));
+    float *U32 = c.matT.as<float>();
+    const long nn = (long)nb * nb;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)std::min<long>((nn + 255) / 256, 1024)), dim3(256),
+                       0, c.stream, U, U32, nn);
+    RS_HIP(hipGetLastError());
+    const float one = 1.0f, zero = 0.0f;
+    RS_BLAS(rocblas_sgemm(c.blas, rocblas_operation_none, rocblas_operation_none, (int)M, nb, nb,
+                          &one, A, (int)M, U32, nb, &zero, Out, (int)M));
+    return kOK;
+}
+
 template <int MODE>
 static int gemm_nn_mode(const float *A, const double *U, int nb, long M, float *Out)
 {
     Context &c = ctx();
+    if constexpr (MODE == 1) return gemm_nn_sgemm(A, U, nb, M, Out);
     constexpr int BN = 3, TN = 32 * BN;
     RS_TRY(c.matT.ensure(sizeof(float) * ((size_t)nb * nb + nb)));
     float *U32 = c.matT.as<float>();

@@ -122,140 +122,129 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
 #pragma unroll
     for (int k = 0; k < (MODE == 0 ? NG : 1); ++k) gacc[k] = 0.0;
 
-    // levels 1..NLEV: register queues shifted once per step (computed values);
-    // level 0 (g, loaded from HBM) is a ring indexed by the unrolled step, so
-    // a request stays in flight for PF steps without a register move
     T q[QTOT][PX];
 #pragma unroll
     for (int k = 0; k < QTOT; ++k)
 #pragma unroll
         for (int px = 0; px < PX; ++px) q[k][px] = T(0);
-    constexpr int D0 = qdepth(0, NLEV);
-    T g0[D0][PX];
+
     const int s0 = z0 - NLEV, s1e = z0 + tz + NLEV;
-    // ring slot of plane p: (p - P0) mod D0, P0 = s0 - (D0 - 1 - PF)
+    // level 0: planes s0 - (depth-1-PF) .. s0 + PF - 1 (the prologue of the queue)
+    {
+        constexpr int D0 = qdepth(0, NLEV);
 #pragma unroll
-    for (int k = 0; k < D0 - 1; ++k) {
-        const int zz = s0 - (D0 - 1 - PF) + k;
-        const T *p = src + (long)wrapz(zz, N3) * plane + rowoff;
+        for (int k = 0; k < D0 - 1; ++k) {
+            const int zz = s0 - (D0 - 1 - PF) + k;
+            const T *p = src + (long)wrapz(zz, N3) * plane + rowoff;
 #pragma unroll
-        for (int px = 0; px < PX; ++px) g0[k][px] = p[px * tpr];
+            for (int px = 0; px < PX; ++px) q[qoff(0, NLEV) + k][px] = p[px * tpr];
+        }
     }
 
     const T tc0 = (T)c0, tc1 = (T)c1, tc2 = (T)c2, tc3 = (T)c3, ts1 = (T)s1;
-    for (int sb = s0; sb < s1e; sb += D0) {
+    for (int s = s0; s < s1e; ++s) {
+        T *Lcur = Lbase + (long)((s - s0) & 1) * NLEV * LS;       // written at the end of this step
+        const T *Lprev = Lbase + (long)(((s - s0) & 1) ^ 1) * NLEV * LS;   // left by step s-1
+        {   // request plane s + PF of g
+            int zn = s + PF;
+            while (zn < 0) zn += N3;
+            while (zn >= N3) zn -= N3;
+            const T *p = src + (long)zn * plane + rowoff;
 #pragma unroll
-        for (int u = 0; u < D0; ++u) {
-            const int s = sb + u;
-            if (s >= s1e) break;
-            // slot of level-0 plane s - k at this step
-#define G0(k) g0[(u + 2 * D0 - 1 - PF - (k)) % D0]
-            T *Lcur = Lbase + (long)((s - s0) & 1) * NLEV * LS;       // written at the end of this step
-            const T *Lprev = Lbase + (long)(((s - s0) & 1) ^ 1) * NLEV * LS;   // left by step s-1
-            {   // request plane s + PF of g
-                int zn = s + PF;
-                while (zn < 0) zn += N3;
-                while (zn >= N3) zn -= N3;
-                const T *p = src + (long)zn * plane + rowoff;
-#pragma unroll
-                for (int px = 0; px < PX; ++px) G0(-PF)[px] = p[px * tpr];
-            }
-#pragma unroll
-            for (int l = 0; l < NLEV; ++l) {
-                // level l+1 at plane s-l-1 on rows [l+1, rows-l-1): level l at planes
-                // s-l-2 .. s-l, x/y of plane s-l-1 from Lprev
-                const T *Lp = Lprev + (long)l * LS;
-                const int qn = qnew(l + 1, NLEV);
-                if (r >= l + 1 && r < rows - l - 1) {
-#pragma unroll
-                    for (int px = 0; px < PX; ++px) {
-                        const int x = cx0 + px * tpr;
-                        const T *lr = Lp + r * W + 1 + x;
-                        const T cen = l == 0 ? G0(1)[px] : q[qnew(l, NLEV) - 1][px];
-                        const T zp = l == 0 ? G0(0)[px] : q[qnew(l, NLEV)][px];
-                        const T zm = l == 0 ? G0(2)[px] : q[qnew(l, NLEV) - 2][px];
-                        T a = tc0 * cen;
-                        a = a + tc1 * (lr[1] + lr[-1]) + tc2 * (lr[W] + lr[-W]) + tc3 * (zp + zm);
-                        T v;
-                        if (l == 0) {
-                            v = cen - a;                          // v_0 = g - A g
-                        } else if (l == 1) {
-                            v = ts1 * a - cen;                    // v_1 = tau(A) v_0
-                        } else {
-                            // v_l = 2 tau(A) v_{l-1} - v_{l-2}; v_{l-2} is level l-1 at this plane
-                            const T prev = q[qnew(l - 1, NLEV) - 2][px];
-                            v = T(2) * (ts1 * a - cen) - prev;
-                        }
-                        q[qn][px] = v;
-                    }
-                }
-            }
-            // all levels at plane zo = s - NLEV
-            const int zo = s - NLEV;
-            if (interior && zo >= z0) {
-                if (MODE == 0) {
-                    T vv[NLEV + 1][PX];
-#pragma unroll
-                    for (int px = 0; px < PX; ++px) vv[0][px] = G0(NLEV)[px];
-#pragma unroll
-                    for (int l = 1; l <= NLEV; ++l)
-#pragma unroll
-                        for (int px = 0; px < PX; ++px) vv[l][px] = q[qout(l, NLEV)][px];
-#pragma unroll
-                    for (int px = 0; px < PX; ++px) {
-                        int e = 0;
-#pragma unroll
-                        for (int j = 1; j <= NLEV; ++j)
-#pragma unroll
-                            for (int k = j; k <= NLEV; ++k)
-                                gacc[e++] += (double)vv[j][px] * (double)vv[k][px];
-#pragma unroll
-                        for (int k = 0; k < NLEV; ++k)
-                            gacc[e++] += (double)vv[0][px] * (double)vv[k][px];
-                    }
-                } else {
-                    T *dst = PG + (long)band * ML + (long)zo * plane + rowoff;
-#pragma unroll
-                    for (int px = 0; px < PX; ++px) {
-                        double x = (double)G0(NLEV)[px];
-#pragma unroll
-                        for (int k = 0; k < NLEV; ++k) x += dk[k] * (double)q[qout(k + 1, NLEV)][px];
-                        if (MODE == 2) {
-                            // P = Pg + beta P in the working precision, as k_cg_newp
-                            T p = (T)x;
-                            if (beta) p = p + (T)bk * dst[px * tpr];
-                            dst[px * tpr] = p;
-                        } else {
-                            dst[px * tpr] = (T)x;
-                        }
-                    }
-                }
-            }
-            // stage the newest plane of levels 0 .. NLEV-1 for the next step
-#pragma unroll
-            for (int l = 0; l < NLEV; ++l) {
-                if (r >= l && r < rows - l) {
-                    T *Lp = Lcur + (long)l * LS;
-#pragma unroll
-                    for (int px = 0; px < PX; ++px) {
-                        const int x = cx0 + px * tpr;
-                        const T v = l == 0 ? G0(0)[px] : q[qnew(l, NLEV)][px];
-                        Lp[r * W + 1 + x] = v;
-                        if (x == 0) Lp[r * W + 1 + N1] = v;
-                        if (x == N1 - 1) Lp[r * W] = v;
-                    }
-                }
-            }
-#undef G0
-            __syncthreads();
-            // shift the queues of levels 1..NLEV by one plane
-#pragma unroll
-            for (int l = 1; l <= NLEV; ++l)
-#pragma unroll
-                for (int k = 0; k < qdepth(l, NLEV) - 1; ++k)
-#pragma unroll
-                    for (int px = 0; px < PX; ++px) q[qoff(l, NLEV) + k][px] = q[qoff(l, NLEV) + k + 1][px];
+            for (int px = 0; px < PX; ++px) q[qoff(0, NLEV) + qdepth(0, NLEV) - 1][px] = p[px * tpr];
         }
+#pragma unroll
+        for (int l = 0; l < NLEV; ++l) {
+            // level l+1 at plane s-l-1 on rows [l+1, rows-l-1): level l at planes
+            // s-l-2 .. s-l (slots qnew-2 .. qnew), x/y of plane s-l-1 from Lprev
+            const T *Lp = Lprev + (long)l * LS;
+            const int qc = qnew(l, NLEV) - 1;
+            const int qn = qnew(l + 1, NLEV);
+            if (r >= l + 1 && r < rows - l - 1) {
+#pragma unroll
+                for (int px = 0; px < PX; ++px) {
+                    const int x = cx0 + px * tpr;
+                    const T *lr = Lp + r * W + 1 + x;
+                    const T cen = q[qc][px];
+                    T a = tc0 * cen;
+                    a = a + tc1 * (lr[1] + lr[-1]) + tc2 * (lr[W] + lr[-W]) +
+                        tc3 * (q[qc + 1][px] + q[qc - 1][px]);
+                    T v;
+                    if (l == 0) {
+                        v = cen - a;                          // v_0 = g - A g
+                    } else if (l == 1) {
+                        v = ts1 * a - cen;                    // v_1 = tau(A) v_0
+                    } else {
+                        // v_l = 2 tau(A) v_{l-1} - v_{l-2}; v_{l-2} is level l-1 at this plane
+                        const T prev = q[qnew(l - 1, NLEV) - 2][px];
+                        v = T(2) * (ts1 * a - cen) - prev;
+                    }
+                    q[qn][px] = v;
+                }
+            }
+        }
+        // all levels at plane zo = s - NLEV
+        const int zo = s - NLEV;
+        if (interior && zo >= z0) {
+            if (MODE == 0) {
+                T vv[NLEV + 1][PX];
+#pragma unroll
+                for (int l = 0; l <= NLEV; ++l)
+#pragma unroll
+                    for (int px = 0; px < PX; ++px) vv[l][px] = q[qout(l, NLEV)][px];
+#pragma unroll
+                for (int px = 0; px < PX; ++px) {
+                    int e = 0;
+#pragma unroll
+                    for (int j = 1; j <= NLEV; ++j)
+#pragma unroll
+                        for (int k = j; k <= NLEV; ++k)
+                            gacc[e++] += (double)vv[j][px] * (double)vv[k][px];
+#pragma unroll
+                    for (int k = 0; k < NLEV; ++k)
+                        gacc[e++] += (double)vv[0][px] * (double)vv[k][px];
+                }
+            } else {
+                T *dst = PG + (long)band * ML + (long)zo * plane + rowoff;
+#pragma unroll
+                for (int px = 0; px < PX; ++px) {
+                    double x = (double)q[qout(0, NLEV)][px];
+#pragma unroll
+                    for (int k = 0; k < NLEV; ++k) x += dk[k] * (double)q[qout(k + 1, NLEV)][px];
+                    if (MODE == 2) {
+                        // P = Pg + beta P in the working precision, as k_cg_newp
+                        T p = (T)x;
+                        if (beta) p = p + (T)bk * dst[px * tpr];
+                        dst[px * tpr] = p;
+                    } else {
+                        dst[px * tpr] = (T)x;
+                    }
+                }
+            }
+        }
+        // stage the newest plane of levels 0 .. NLEV-1 for the next step
+#pragma unroll
+        for (int l = 0; l < NLEV; ++l) {
+            if (r >= l && r < rows - l) {
+                T *Lp = Lcur + (long)l * LS;
+#pragma unroll
+                for (int px = 0; px < PX; ++px) {
+                    const int x = cx0 + px * tpr;
+                    const T v = q[qnew(l, NLEV)][px];
+                    Lp[r * W + 1 + x] = v;
+                    if (x == 0) Lp[r * W + 1 + N1] = v;
+                    if (x == N1 - 1) Lp[r * W] = v;
+                }
+            }
+        }
+        __syncthreads();
+        // shift every queue by one plane
+#pragma unroll
+        for (int l = 0; l <= NLEV; ++l)
+#pragma unroll
+            for (int k = 0; k < qdepth(l, NLEV) - 1; ++k)
+#pragma unroll
+                for (int px = 0; px < PX; ++px) q[qoff(l, NLEV) + k][px] = q[qoff(l, NLEV) + k + 1][px];
     }
     if (MODE == 0) {
         const int lane = tid & 63, wave = tid >> 6, nw = (blockDim.x + 63) >> 6;
