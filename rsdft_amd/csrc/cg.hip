@@ -77,6 +77,29 @@ __device__ __forceinline__ double sum_parts(const double *part, int nchunk) {
     return t;
 }
 
+// Collapse the per-chunk partial sums of `rows` rows (row r at part[r*nchunk])
+// into part[r*nchunk]: one wave per row, lanes take strided chunks, then a
+// fixed shuffle tree (deterministic).  The per-band scalar kernels that
+// follow read one value per row (nchunk = 1).
+__global__ void __launch_bounds__(256) k_collapse_parts(double *part, long rows, int nchunk)
+{
+    const long r = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= rows) return;
+    double *p = part + r * nchunk;
+    double t = 0.0;
+    for (int c = lane; c < nchunk; c += 64) t += p[c];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    if (lane == 0) p[0] = t;
+}
+
+static void collapse_parts(double *part, long rows, int nchunk)
+{
+    if (nchunk <= 1) return;
+    hipLaunchKernelGGL(k_collapse_parts, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0,
+                       ctx().stream, part, rows, nchunk);
+}
+
 __global__ void k_sum_parts(const double *part, int nb, int nchunk, int nd, double scale,
                             double *out, const int *active)
 {
@@ -139,7 +162,7 @@ __global__ void k_cg_init(CGView v, const double *part, int nb, int nchunk, doub
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    v.E[b] = dV * sum_parts(part + (long)b * nchunk, nchunk);
+    v.E[b] = dV * part[(long)b * nchunk];   // collapsed partials
     v.E1[b] = 1.0e10;
     v.active[b] = 1;
 }
@@ -149,7 +172,7 @@ __global__ void k_cg_rb(CGView v, const double *part, int nb, int nchunk, double
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb || !v.active[b]) return;
-    v.rb[b] = dV * sum_parts(part + (long)b * nchunk, nchunk);
+    v.rb[b] = dV * part[(long)b * nchunk];
 }
 
 // top of the icg loop (cg_module.f90:213-219): res = rb/c1^2, early exits
@@ -167,7 +190,7 @@ __global__ void k_cg_pgg(CGView v, const double *part, int nb, int nchunk, doubl
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb || !v.active[b]) return;
-    const double rb = dV * sum_parts(part + (long)b * nchunk, nchunk);
+    const double rb = dV * part[(long)b * nchunk];
     v.rb[b] = rb;
     v.bk[b] = first ? 0.0 : rb / v.gkgk[b];
     v.gkgk[b] = rb;
@@ -190,7 +213,7 @@ __global__ void k_cg_2x2(CGView v, const double *part, int nb, int nchunk, doubl
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb || !v.active[b]) return;
     double w[6];
-    for (int k = 0; k < 6; ++k) w[k] = dV * sum_parts(part + ((long)k * nb + b) * nchunk, nchunk);
+    for (int k = 0; k < 6; ++k) w[k] = dV * part[((long)k * nb + b) * nchunk];
     const double Bm[4] = {w[0], w[1], w[1], w[2]};
     const double Am[4] = {w[3], w[4], w[4], w[5]};
     double W[2], Z[4];
@@ -212,7 +235,7 @@ __global__ void k_cg_reject(CGView v, const double *part, int nb, int nchunk, do
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb || !v.active[b]) return;
-    const double rb = dV * sum_parts(part + (long)b * nchunk, nchunk);
+    const double rb = dV * part[(long)b * nchunk];
     v.rb[b] = rb;
     if (rb / v.res[b] > 1.0e8) {
         v.E[b] = v.E1[b];
@@ -302,10 +325,11 @@ k_cg_newp(const T *PG, T *P, CGView v, long ML, int first)
 }
 
 // Hx = u1 Hx + u2 Hp; g = -2 (Hx - W (u1 x + u2 p)); part = sum g^2
-// (cg_module.f90:337-349).  When u1 > 1/2 the new x = u1 x + u2 p is stored
+// (cg_module.f90:337-349).  When u1 > 1/20 the new x = u1 x + u2 p is stored
 // here already (the reject rule of cg_module.f90:360-368 almost never fires;
-// k_cg_accept undoes it then), which saves a pass over x and p.
-__device__ __forceinline__ bool cg_speculative(double u1) { return u1 > 0.5; }
+// k_cg_accept undoes it then, losing at most a factor 20 in rounding), which
+// saves a pass over x and p.
+__device__ __forceinline__ bool cg_speculative(double u1) { return u1 > 0.05; }
 
 template <class T>
 __global__ void __launch_bounds__(kChunkThreads)
@@ -330,21 +354,23 @@ k_cg_update(T *HX, const T *HP, T *X, const T *P, T *G, CGView v, long ML, int n
 }
 
 // x = u1 x + u2 p unless rejected (cg_module.f90:366-368); for the bands
-// k_cg_update already moved, a rejection restores x = (x_new - u2 p) / u1
+// k_cg_update already moved, a rejection restores x = (x_new - u2 p) / u1.
+// One workgroup per band: almost every band has nothing to do here.
 template <class T>
-__global__ void __launch_bounds__(kChunkThreads)
+__global__ void __launch_bounds__(1024)
 k_cg_accept(T *X, const T *P, CGView v, long ML)
 {
-    BAND_LOOP_BEGIN
+    const int b = blockIdx.x;
     if (!v.active[b]) return;
     const bool spec = cg_speculative(v.u1[b]);
     if (spec == (v.accept[b] != 0)) return;
     const T u1 = (T)v.u1[b], u2 = (T)v.u2[b];
+    const long off = (long)b * ML;
     if (spec) {
-        for (long i = i0 + threadIdx.x; i < i1; i += kChunkThreads)
+        for (long i = threadIdx.x; i < ML; i += blockDim.x)
             X[off + i] = (X[off + i] - u2 * P[off + i]) / u1;
     } else {
-        for (long i = i0 + threadIdx.x; i < i1; i += kChunkThreads)
+        for (long i = threadIdx.x; i < ML; i += blockDim.x)
             X[off + i] = u1 * X[off + i] + u2 * P[off + i];
     }
 }
@@ -512,9 +538,11 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
     {
         const T *a[1] = {X}, *bb[1] = {HX};
         RS_TRY(band_dots<T>(1, a, bb, nb, 1.0, nullptr, nullptr));
+        collapse_parts(part, nb, nch);
         hipLaunchKernelGGL(k_cg_init, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV);
     }
     hipLaunchKernelGGL((k_cg_grad<T>), egrid, eblk, 0, c.stream, HX, X, G, v, ML, nch, part);
+    collapse_parts(part, nb, nch);
     hipLaunchKernelGGL(k_cg_rb, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV);
     for (int icg = 1; icg <= c.ncg + 1; ++icg) {
         hipLaunchKernelGGL(k_cg_top, bgrid, bblk, 0, c.stream, v, nb);
@@ -531,6 +559,7 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
             {
                 const T *a[1] = {PG}, *bb[1] = {G};
                 RS_TRY(band_dots<T>(1, a, bb, nb, 1.0, nullptr, act));
+                collapse_parts(part, nb, nch);
                 hipLaunchKernelGGL(k_cg_pgg, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV,
                                    icg == 1 ? 1 : 0);
             }
@@ -541,12 +570,14 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
         {
             const T *a[6] = {X, P, P, X, P, P}, *bb[6] = {X, X, P, HX, HX, HP};
             RS_TRY(band_dots<T>(6, a, bb, nb, 1.0, nullptr, act));
+            collapse_parts(part, 6L * nb, nch);
             hipLaunchKernelGGL(k_cg_2x2, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV);
         }
         hipLaunchKernelGGL((k_cg_update<T>), egrid, eblk, 0, c.stream, HX, HP, X, P, G, v, ML,
                            nch, part);
+        collapse_parts(part, nb, nch);
         hipLaunchKernelGGL(k_cg_reject, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV);
-        hipLaunchKernelGGL((k_cg_accept<T>), egrid, eblk, 0, c.stream, X, P, v, ML);
+        hipLaunchKernelGGL((k_cg_accept<T>), dim3(nb), dim3(1024), 0, c.stream, X, P, v, ML);
     }
     hipLaunchKernelGGL(k_cg_final, bgrid, bblk, 0, c.stream, v, nb, esp_dev, res_dev);
     RS_HIP(hipGetLastError());
