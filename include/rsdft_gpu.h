@@ -72,9 +72,19 @@ RSDFT_API int rsdft_gpu_set_precond_mode(int mode);
 
 /* Eigensolver of the subspace_diag step (subspace_diag_la_module.f90:159-169):
  * 1 (default) quadratically convergent refinement from the identity when
- * Hsub is nearly diagonal, falling back to rocsolver_dsyevd otherwise;
- * 0 -- always rocsolver_dsyevd. */
+ * Hsub is nearly diagonal, otherwise mode 2; 2 -- Householder reduction to
+ * tridiagonal form in one cooperative launch with Hsub resident in LDS, then
+ * rocsolver_dstedc and rocsolver_dormtr (rocsolver_dsyevd when Hsub does not
+ * fit the LDS of the device); 0 -- always rocsolver_dsyevd. */
 RSDFT_API int rsdft_gpu_set_eigensolver(int mode);
+
+/* How rsdft_gpu_sweep runs gram_schmidt + subspace_diag after CG
+ * (sweep_module.f90:147-158): 1 (default) fused -- R^-1 of the Cholesky
+ * factor of dV X^T X, Hsub = R^-T (0.5 dV (X^T HX + HX^T X)) R^-1 from the
+ * H x that CG leaves (recomputed only for bands with a rejected step) and
+ * one rotation psi <- X R^-1 V; equal to the two steps in exact arithmetic,
+ * one H psi and one rotation fewer.  0 -- the two steps as in the reference. */
+RSDFT_API int rsdft_gpu_set_sweep_mode(int mode);
 
 /* hamiltonian(k,s,tpsi,htpsi,n1,n2,ib1,ib2) (hamiltonian_module.f90:20-72):
  * htpsi = T tpsi + Vloc tpsi + V_NL tpsi (htpsi overwritten).  Gamma only:
@@ -144,6 +154,14 @@ RSDFT_API int rsdft_gpu_band_energy_grad(const double *unk, int ML0, int MB, con
  * compact buffer; FP64: scatter-add), [7] a plain copy of all bands (the
  * bandwidth reference). */
 RSDFT_API int rsdft_gpu_test_hpsi(int nrep, double *times_ms);
+
+/* Diagnostics of the subspace eigensolve (the dsyevd call of
+ * subspace_diag_la_module.f90:159-169): eigenpairs of the symmetric n x n
+ * host matrix H (column-major, full storage) by the eigensolver `mode` of
+ * rsdft_gpu_set_eigensolver; w[n] ascending, V (n x n, may be NULL) with
+ * V(j,j) >= 0, *ms (may be NULL) the device time of the solve. */
+RSDFT_API int rsdft_gpu_test_eigh(const double *H, int n, int mode, double *w, double *V,
+                                  double *ms);
 
 /* Per-phase timers in milliseconds since the last reset (watch_module.f90
  * time_hmlt/time_cgpc analogue): [0] hamiltonian, [1] preconditioner,

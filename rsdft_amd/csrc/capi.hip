@@ -176,12 +176,25 @@ static int run_sweeps(int MB, int nsweep, double *esp_dev, double *res_dev, Phas
         RS_TRY(pe.begin(kTCg));
         RS_TRY(cg_batched<T>(psiT<T>(), MB, esp_dev, res_dev));
         RS_TRY(pe.end());
-        RS_TRY(pe.begin(kTGs));
-        RS_TRY(gram_schmidt_dev<T>(psiT<T>(), MB));
-        RS_TRY(pe.end());
-        RS_TRY(pe.begin(kTSd));
-        RS_TRY(subspace_diag_dev<T>(psiT<T>(), MB, esp_dev));
-        RS_TRY(pe.end());
+        if (ctx().sweep_mode == 1) {
+            // CG leaves H x in c.hx except for bands with a rejected step
+            // (there H x moved and x did not): recompute those bands only
+            T *HX = ctx().hx.as<T>();
+            RS_TRY(pe.begin(kTGs));
+            RS_TRY(hpsi_apply<T>(psiT<T>(), HX, MB, kKin | kLoc | kNl, ctx().cg.rejected));
+            RS_TRY(cholesky_inverse_dev<T>(psiT<T>(), MB));
+            RS_TRY(pe.end());
+            RS_TRY(pe.begin(kTSd));
+            RS_TRY(rayleigh_ritz_factored_dev<T>(psiT<T>(), HX, MB, esp_dev));
+            RS_TRY(pe.end());
+        } else {
+            RS_TRY(pe.begin(kTGs));
+            RS_TRY(gram_schmidt_dev<T>(psiT<T>(), MB));
+            RS_TRY(pe.end());
+            RS_TRY(pe.begin(kTSd));
+            RS_TRY(subspace_diag_dev<T>(psiT<T>(), MB, esp_dev));
+            RS_TRY(pe.end());
+        }
         // close the sweep pair (it was opened first)
         RS_HIP(hipEventRecord(pe.ev[pe.ev.size() - 4].second, ctx().stream));
     }
@@ -493,8 +506,15 @@ int rsdft_gpu_set_precond_mode(int mode)
 
 int rsdft_gpu_set_eigensolver(int mode)
 {
-    if (mode < 0 || mode > 1) return set_error(kErrArg, "eigensolver mode %d", mode);
+    if (mode < 0 || mode > 2) return set_error(kErrArg, "eigensolver mode %d", mode);
     ctx().eig_mode = mode;
+    return kOK;
+}
+
+int rsdft_gpu_set_sweep_mode(int mode)
+{
+    if (mode < 0 || mode > 1) return set_error(kErrArg, "sweep mode %d", mode);
+    ctx().sweep_mode = mode;
     return kOK;
 }
 
@@ -788,6 +808,41 @@ int rsdft_gpu_test_hpsi(int nrep, double *times_ms)
     return kOK;
 }
 
+int rsdft_gpu_test_eigh(const double *H, int n, int mode, double *w, double *V, double *ms)
+{
+    RS_TRY(need_init());
+    if (!H || !w || n < 1 || mode < 0 || mode > 2)
+        return set_error(kErrArg, "test_eigh: n=%d mode=%d", n, mode);
+    Context &c = ctx();
+    const size_t nn = (size_t)n * n;
+    RS_TRY(c.mat2.ensure(sizeof(double) * nn));
+    RS_TRY(c.mat3.ensure(sizeof(double) * nn));
+    RS_TRY(c.eval.ensure(sizeof(double) * 2 * (size_t)n));
+    RS_TRY(clear_solver_flags());
+    double *A = c.mat2.as<double>(), *Vb = c.mat3.as<double>(), *D = c.eval.as<double>(), *Vec;
+    RS_HIP(hipMemcpyAsync(A, H, sizeof(double) * nn, hipMemcpyHostToDevice, c.stream));
+    hipEvent_t a, b;
+    RS_HIP(hipEventCreate(&a));
+    RS_HIP(hipEventCreate(&b));
+    const int saved = c.eig_mode;
+    c.eig_mode = mode;
+    RS_HIP(hipEventRecord(a, c.stream));
+    const int rc = eigh_sub(A, n, Vb, D, &Vec);
+    c.eig_mode = saved;
+    RS_TRY(rc);
+    RS_HIP(hipEventRecord(b, c.stream));
+    RS_HIP(hipMemcpyAsync(w, D, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    if (V) RS_HIP(hipMemcpyAsync(V, Vec, sizeof(double) * nn, hipMemcpyDeviceToHost, c.stream));
+    RS_HIP(hipEventSynchronize(b));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    float t = 0.0f;
+    RS_HIP(hipEventElapsedTime(&t, a, b));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return check_solver_flags();
+}
+
 int rsdft_gpu_get_timers(double *t, int n)
 {
     Context &c = ctx();
@@ -814,7 +869,7 @@ int rsdft_gpu_finalize(void)
     (void)hipStreamSynchronize(c.stream);
     DevMem *all[] = {&c.vloc64, &c.vloc32, &c.psi, &c.hx, &c.hp, &c.gk, &c.pgk, &c.pk, &c.rk,
                      &c.ppc, &c.qpc, &c.tmp, &c.coefC, &c.part, &c.stage, &c.mat, &c.mat2,
-                     &c.mat3, &c.eval, &c.einfo, &c.evwork, &c.matT, &c.dots, &c.gemm_part, &c.opbuf, &c.pc_coef,
+                     &c.mat3, &c.mat4, &c.eval, &c.einfo, &c.evwork, &c.matT, &c.dots, &c.gemm_part, &c.opbuf, &c.pc_coef,
                      &c.cg.buf, &c.cg.flags, &c.nl.g_lma, &c.nl.g_ptr, &c.nl.g_idx, &c.nl.g_val64,
                      &c.nl.g_val32, &c.nl.g_order, &c.nl.scale64, &c.nl.d_ptr, &c.nl.d_idx, &c.nl.d_val,
                      &c.nl.cov_index, &c.nl.mem_ptr, &c.nl.mem_pos, &c.nl.mem_g, &c.nlbuf,
@@ -837,6 +892,7 @@ int rsdft_gpu_finalize(void)
     c.nl.natom = 0;
     c.pc_mode = 1;
     c.eig_mode = 1;
+    c.sweep_mode = 1;
     c.eig_refined = 0;
     for (double &x : c.timers.ms) x = 0.0;
     c.hpsi_bands = 0;

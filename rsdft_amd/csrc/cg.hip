@@ -149,12 +149,12 @@ __device__ __host__ inline void sygv2(const double *A, const double *B, double *
 // ---------------------------------------------------------------------------
 struct CGView {
     double *E, *E1, *rb, *res, *gkgk, *bk, *u1, *u2, *W, *rr0, *rr1, *pa, *pb;
-    int *active, *accept, *pc_active;
+    int *active, *accept, *pc_active, *rejected;
 };
 
 static CGView view(CGScalars &s) {
     return CGView{s.E, s.E1, s.rb, s.res, s.gkgk, s.bk, s.u1, s.u2, s.W, s.rr0, s.rr1,
-                  s.pa, s.pb, s.active, s.accept, s.pc_active};
+                  s.pa, s.pb, s.active, s.accept, s.pc_active, s.rejected};
 }
 
 // E = dV <x|Hx>, E1 = 1e10 (cg_module.f90:162,176-182)
@@ -165,6 +165,7 @@ __global__ void k_cg_init(CGView v, const double *part, int nb, int nchunk, doub
     v.E[b] = dV * part[(long)b * nchunk];   // collapsed partials
     v.E1[b] = 1.0e10;
     v.active[b] = 1;
+    v.rejected[b] = 0;
 }
 
 // rb = dV <g|g>
@@ -240,6 +241,7 @@ __global__ void k_cg_reject(CGView v, const double *part, int nb, int nchunk, do
     if (rb / v.res[b] > 1.0e8) {
         v.E[b] = v.E1[b];
         v.accept[b] = 0;
+        v.rejected[b] = 1;   // Hx moved, x did not
     } else {
         v.accept[b] = 1;
     }
@@ -425,7 +427,7 @@ __global__ void k_copy_active(const T *src, T *dst, long ML, const int *active)
 int CGScalars::ensure(int MB)
 {
     RS_TRY(buf.ensure(sizeof(double) * 13 * (size_t)MB));
-    RS_TRY(flags.ensure(sizeof(int) * 3 * (size_t)MB));
+    RS_TRY(flags.ensure(sizeof(int) * 4 * (size_t)MB));
     double *d = buf.as<double>();
     double **slots[13] = {&E, &E1, &rb, &res, &gkgk, &bk, &u1, &u2, &W, &rr0, &rr1, &pa, &pb};
     for (int k = 0; k < 13; ++k) *slots[k] = d + (size_t)k * MB;
@@ -433,6 +435,7 @@ int CGScalars::ensure(int MB)
     active = f;
     accept = f + MB;
     pc_active = f + 2 * (size_t)MB;
+    rejected = f + 3 * (size_t)MB;
     return kOK;
 }
 

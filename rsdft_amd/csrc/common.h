@@ -135,6 +135,7 @@ struct CGScalars {
     DevMem flags;     // int arrays
     double *E, *E1, *rb, *res, *gkgk, *bk, *u1, *u2, *W, *rr0, *rr1, *pa, *pb;
     int *active, *accept, *pc_active;
+    int *rejected;    // 1 when a step of this band was rejected in the last CG call
     int ensure(int MB);
 };
 
@@ -155,7 +156,8 @@ struct Context {
     int ncg = 3, mloop = 3;
     int pc_mode = 1;   // 0: step-by-step preconditioner recurrence, 1: s-step form
     size_t max_lds = 64 * 1024;   // LDS bytes one workgroup may allocate on this device
-    int eig_mode = 1;  // 0: always dsyevd, 1: refinement when Hsub is nearly diagonal
+    int eig_mode = 1;  // 0: dsyevd, 1: refinement when Hsub is nearly diagonal, else 2: own tridiagonal reduction
+    int sweep_mode = 1;  // 1: Rayleigh-Ritz through the Cholesky factor and CG's H x, 0: GS then subspace_diag
     long eig_refined = 0;
 
     DevMem vloc64, vloc32;
@@ -171,7 +173,7 @@ struct Context {
     DevMem part;       // reduction partial sums
     DevMem stage;      // host<->device staging (FP64)
     DevMem opbuf;      // working-precision staging of the op_* entry points
-    DevMem mat, mat2, mat3, eval, einfo, evwork;   // nb x nb FP64 dense work
+    DevMem mat, mat2, mat3, mat4, eval, einfo, evwork;   // nb x nb FP64 dense work
     DevMem matT;       // nb x nb in working precision
     DevMem dots;       // generic per-band result buffer (FP64)
     DevMem pc_coef;    // s-step preconditioner coefficients [band][mloop]
@@ -244,6 +246,18 @@ template <class T>
 int gram_schmidt_dev(T *x, int nb);
 template <class T>
 int subspace_diag_dev(T *x, int nb, double *esp_dev);
+// Eigenpairs of the symmetric nb x nb H (destroyed) by the eig_mode solver:
+// eigenvalues ascending in D (2 nb doubles of room), eigenvectors with
+// V(j,j) >= 0 at *Vec (H or Vbuf); failures flagged in c.einfo[3]
+int eigh_sub(double *H, int nb, double *Vbuf, double *D, double **Vec);
+// The two steps of one sweep after CG, fused: R^-1 of the Cholesky factor of
+// dV x^T x into c.mat, then the Ritz pairs of (x R^-1) from x and hx = H x,
+// psi <- x R^-1 V (the Gram-Schmidt and subspace_diag results in exact
+// arithmetic, without a second H psi or a second rotation)
+template <class T>
+int cholesky_inverse_dev(const T *x, int nb);
+template <class T>
+int rayleigh_ritz_factored_dev(T *x, const T *hx, int nb, double *esp_dev);
 
 // adjoint
 template <class T>

@@ -17,10 +17,11 @@
 
 namespace rsdft {
 
+int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *done);
 template <class T>
 int split_gemm_tn(const float *A, const float *B, int nb, long K, double alpha, double *C);
 template <class T>
-int split_gemm_nn(const float *A, const double *U, int nb, long M, float *C);
+int split_gemm_nn(const float *A, const double *U, int nb, long M, float *C, int fp32);
 
 template <class S, class D>
 __global__ void k_convert(const S *src, D *dst, long n)
@@ -513,9 +514,10 @@ static int gemm_tn(const T *A, const T *B, int nb, double alpha, double *C)
     return split_gemm_tn<T>((const float *)A, (const float *)B, nb, ML, alpha, C);
 }
 
-// Out(ML x nb) = A(ML x nb) * U(nb x nb), U FP64
+// Out(ML x nb) = A(ML x nb) * U(nb x nb), U FP64; fp32: FP32 products in the
+// BF16 split mode too
 template <class T>
-static int gemm_nn(const T *A, const double *U, int nb, T *Out)
+static int gemm_nn(const T *A, const double *U, int nb, T *Out, int fp32 = 0)
 {
     Context &c = ctx();
     const long ML = c.g.ML;
@@ -526,7 +528,7 @@ static int gemm_nn(const T *A, const double *U, int nb, T *Out)
                               (int)ML));
         return kOK;
     }
-    return split_gemm_nn<T>((const float *)A, U, nb, ML, (float *)Out);
+    return split_gemm_nn<T>((const float *)A, U, nb, ML, (float *)Out, fp32);
 }
 
 static int ensure_dense(int nb)
@@ -534,8 +536,53 @@ static int ensure_dense(int nb)
     Context &c = ctx();
     RS_TRY(c.mat.ensure(sizeof(double) * (size_t)nb * nb));
     RS_TRY(c.mat2.ensure(sizeof(double) * (size_t)nb * nb));
+    RS_TRY(c.mat3.ensure(sizeof(double) * (size_t)nb * nb));
+    RS_TRY(c.mat4.ensure(sizeof(double) * (size_t)nb * nb));
     RS_TRY(c.eval.ensure(sizeof(double) * (size_t)nb * 2));
     RS_TRY(c.einfo.ensure(sizeof(int) * 4));
+    return kOK;
+}
+
+// S <- R^-1 in place, S = dV X^T X = R^T R on entry (upper Cholesky factor)
+static int chol_inverse(double *S, int nb)
+{
+    Context &c = ctx();
+    int *info = c.einfo.as<int>();
+    RS_BLAS(rocsolver_dpotrf(c.blas, rocblas_fill_upper, nb, S, nb, info));
+    hipLaunchKernelGGL(k_check_info, dim3(1), dim3(1), 0, c.stream, info, info + 2);
+    RS_BLAS(rocsolver_dtrtri(c.blas, rocblas_fill_upper, rocblas_diagonal_non_unit, nb, S, nb,
+                             info + 1));
+    hipLaunchKernelGGL(k_zero_lower, dim3((nb + 127) / 128, nb), dim3(128), 0, c.stream, S, nb);
+    RS_HIP(hipGetLastError());
+    return kOK;
+}
+
+// Eigenpairs of the symmetric Hsub H (destroyed): eigenvalues ascending in D,
+// eigenvectors (columns, V(j,j) >= 0) at *Vec, which is H or Vbuf.
+// eig_mode 1: refinement when H is nearly diagonal, else the LDS-resident
+// tridiagonal reduction with rocsolver_dstedc / dormtr; 2: the latter always;
+// 0 (and matrices too large for the reduction): rocsolver_dsyevd.
+int eigh_sub(double *H, int nb, double *Vbuf, double *D, double **Vec)
+{
+    Context &c = ctx();
+    int *info = c.einfo.as<int>();
+    int refined = 0, done = 0;
+    *Vec = H;
+    if (c.eig_mode == 1) {
+        RS_TRY(refine_eigh(H, nb, Vbuf, D, &refined));
+        c.eig_refined += refined;
+        done = refined;
+    }
+    if (!done && c.eig_mode != 0) RS_TRY(eigh_tridiag(H, nb, Vbuf, D, info, &done));
+    if (done) {
+        *Vec = Vbuf;
+    } else {
+        RS_BLAS(rocsolver_dsyevd(c.blas, rocblas_evect_original, rocblas_fill_upper, nb, H, nb, D,
+                                 D + nb, info));
+        hipLaunchKernelGGL(k_check_info, dim3(1), dim3(1), 0, c.stream, info, info + 3);
+    }
+    hipLaunchKernelGGL(k_fix_signs, dim3(nb), dim3(128), 0, c.stream, *Vec, nb);
+    RS_HIP(hipGetLastError());
     return kOK;
 }
 
@@ -546,14 +593,8 @@ int gram_schmidt_dev(T *X, int nb)
     Context &c = ctx();
     RS_TRY(ensure_dense(nb));
     double *S = c.mat.as<double>();
-    int *info = c.einfo.as<int>();
     RS_TRY(gemm_tn<T>(X, X, nb, c.g.dV, S));
-    RS_BLAS(rocsolver_dpotrf(c.blas, rocblas_fill_upper, nb, S, nb, info));
-    hipLaunchKernelGGL(k_check_info, dim3(1), dim3(1), 0, c.stream, info, info + 2);
-    RS_BLAS(rocsolver_dtrtri(c.blas, rocblas_fill_upper, rocblas_diagonal_non_unit, nb, S, nb,
-                             info + 1));
-    hipLaunchKernelGGL(k_zero_lower, dim3((nb + 127) / 128, nb), dim3(128), 0, c.stream, S, nb);
-    RS_HIP(hipGetLastError());
+    RS_TRY(chol_inverse(S, nb));
     T *Out = c.tmp.as<T>();
     RS_TRY(gemm_nn<T>(X, S, nb, Out));
     std::swap(c.psi.p, c.tmp.p);
@@ -574,27 +615,56 @@ int subspace_diag_dev(T *X, int nb, double *esp_dev)
     hipLaunchKernelGGL(k_symmetrize, dim3((nb + 127) / 128, nb), dim3(128), 0, c.stream, C, H, nb,
                        0.5 * c.g.dV);
     RS_HIP(hipGetLastError());
-    int *info = c.einfo.as<int>();
-    double *D = c.eval.as<double>();
-    int refined = 0;
-    double *Vec = H;
-    if (c.eig_mode == 1) {
-        RS_TRY(c.mat3.ensure(sizeof(double) * (size_t)nb * nb));
-        RS_TRY(refine_eigh(H, nb, c.mat3.as<double>(), D, &refined));
-        if (refined) {
-            Vec = c.mat3.as<double>();
-            hipLaunchKernelGGL(k_fix_signs, dim3(nb), dim3(128), 0, c.stream, Vec, nb);
-        }
-        c.eig_refined += refined;
-    }
-    if (!refined) {
-        RS_BLAS(rocsolver_dsyevd(c.blas, rocblas_evect_original, rocblas_fill_upper, nb, H, nb, D,
-                                 D + nb, info));
-        hipLaunchKernelGGL(k_check_info, dim3(1), dim3(1), 0, c.stream, info, info + 3);
-        hipLaunchKernelGGL(k_fix_signs, dim3(nb), dim3(128), 0, c.stream, H, nb);
-    }
+    double *D = c.eval.as<double>(), *Vec = nullptr;
+    RS_TRY(eigh_sub(H, nb, c.mat3.as<double>(), D, &Vec));
     T *Out = c.tmp.as<T>();
     RS_TRY(gemm_nn<T>(X, Vec, nb, Out));
+    RS_HIP(hipMemcpyAsync(esp_dev, D, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
+    std::swap(c.psi.p, c.tmp.p);
+    std::swap(c.psi.bytes, c.tmp.bytes);
+    return kOK;
+}
+
+// c.mat <- R^-1, R^T R = dV X^T X
+template <class T>
+int cholesky_inverse_dev(const T *X, int nb)
+{
+    Context &c = ctx();
+    RS_TRY(ensure_dense(nb));
+    double *S = c.mat.as<double>();
+    RS_TRY(gemm_tn<T>(X, X, nb, c.g.dV, S));
+    return chol_inverse(S, nb);
+}
+
+// With Q = X R^-1 (R^-1 in c.mat from cholesky_inverse_dev):
+//   Hsub = 0.5 dV (Q^T H Q + (Q^T H Q)^T) = R^-T [0.5 dV (C + C^T)] R^-1,
+//   C = X^T HX, then psi <- Q V = X (R^-1 V) in one rotation.
+template <class T>
+int rayleigh_ritz_factored_dev(T *X, const T *HX, int nb, double *esp_dev)
+{
+    Context &c = ctx();
+    RS_TRY(ensure_dense(nb));
+    double *Ri = c.mat.as<double>(), *C = c.mat2.as<double>(), *H = c.mat3.as<double>(),
+           *V4 = c.mat4.as<double>();
+    const double one = 1.0, zero = 0.0;
+    RS_TRY(gemm_tn<T>(X, HX, nb, 1.0, C));
+    hipLaunchKernelGGL(k_symmetrize, dim3((nb + 127) / 128, nb), dim3(128), 0, c.stream, C, H, nb,
+                       0.5 * c.g.dV);
+    RS_HIP(hipGetLastError());
+    // C <- Hs R^-1, H <- R^-T C
+    RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_none, nb, nb, nb, &one,
+                          H, nb, Ri, nb, &zero, C, nb));
+    RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, nb,
+                          &one, Ri, nb, C, nb, &zero, H, nb));
+    double *D = c.eval.as<double>(), *Vec = nullptr;
+    RS_TRY(eigh_sub(H, nb, V4, D, &Vec));
+    // C <- R^-1 V
+    RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_none, nb, nb, nb, &one,
+                          Ri, nb, Vec, nb, &zero, C, nb));
+    // FP32 products: an error of the rotation here is not absorbed by a later
+    // Rayleigh-Ritz step (the BF16 split product of the two-step path is)
+    T *Out = c.tmp.as<T>();
+    RS_TRY(gemm_nn<T>(X, C, nb, Out, 1));
     RS_HIP(hipMemcpyAsync(esp_dev, D, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
     std::swap(c.psi.p, c.tmp.p);
     std::swap(c.psi.bytes, c.tmp.bytes);
@@ -609,5 +679,9 @@ template int gram_schmidt_dev<double>(double *, int);
 template int gram_schmidt_dev<float>(float *, int);
 template int subspace_diag_dev<double>(double *, int, double *);
 template int subspace_diag_dev<float>(float *, int, double *);
+template int cholesky_inverse_dev<double>(const double *, int);
+template int cholesky_inverse_dev<float>(const float *, int);
+template int rayleigh_ritz_factored_dev<double>(double *, const double *, int, double *);
+template int rayleigh_ritz_factored_dev<float>(float *, const float *, int, double *);
 
 }  // namespace rsdft

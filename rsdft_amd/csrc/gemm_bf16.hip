@@ -443,15 +443,17 @@ int split_gemm_tn(const float *A, const float *B, int nb, long K, double alpha, 
                            : gemm_tn_mode<1>(A, B, nb, K, alpha, C);
 }
 
+// fp32 != 0: FP32 products whatever the precision mode
 template <class T>
-int split_gemm_nn(const float *A, const double *U, int nb, long M, float *Out)
+int split_gemm_nn(const float *A, const double *U, int nb, long M, float *Out, int fp32)
 {
-    return ctx().prec == 2 ? gemm_nn_mode<2>(A, U, nb, M, Out) : gemm_nn_mode<1>(A, U, nb, M, Out);
+    return ctx().prec == 2 && !fp32 ? gemm_nn_mode<2>(A, U, nb, M, Out)
+                                    : gemm_nn_mode<1>(A, U, nb, M, Out);
 }
 
 template int split_gemm_tn<float>(const float *, const float *, int, long, double, double *);
 template int split_gemm_tn<double>(const float *, const float *, int, long, double, double *);
-template int split_gemm_nn<float>(const float *, const double *, int, long, float *);
-template int split_gemm_nn<double>(const float *, const double *, int, long, float *);
+template int split_gemm_nn<float>(const float *, const double *, int, long, float *, int);
+template int split_gemm_nn<double>(const float *, const double *, int, long, float *, int);
 
 }  // namespace rsdft

@@ -1,0 +1,357 @@
+// Symmetric eigensolver of the subspace matrix: Householder reduction to
+// tridiagonal form in one cooperative launch, rocsolver_dstedc on the
+// tridiagonal, and the back-transformation by a wave-per-columns kernel.
+//
+// The reduction is dsytd2 with the lower convention: A = Q T Q^T,
+// Q = H(0) H(1) ... H(n-2), H(i) = I - tau_i v_i v_i^T with v_i(0:i) = 0,
+// v_i(i+1) = 1 and v_i(i+2:n) stored in R(i+2:n, i); D, E the diagonal and
+// subdiagonal of T (the output format of rocsolver_dsytrd).  At n = 480
+// rocsolver_dsytrd takes ~7.5 ms, almost all of it in ~1,500 small launches;
+// here the matrix stays in LDS and the reduction takes ~2.5 ms.
+//
+// nwg workgroups own the columns cyclically (column j in workgroup j % nwg,
+// local slot j / nwg) and a column step needs one grid barrier:
+//   every workgroup holds v_k, tau_k (computed redundantly, identically),
+//   a. p_j = tau_k A(k+1:n, j)^T v_k for its columns j > k (A is symmetric,
+//      so this is entry j of tau_k A v_k); it publishes p_j; the owner of
+//      column k+1 publishes that column;
+//   -- grid barrier --
+//   b. every workgroup reads the whole p, forms p^T v_k itself (the same
+//      fixed-order sum everywhere), w = p - (tau_k / 2)(p^T v_k) v_k, and
+//      A(k+1:n, j) -= v_k w_j + w v_k(j) on its columns; it applies the same
+//      update to the published column k+1 and forms v_{k+1}, tau_{k+1} from
+//      it (dlarfg).
+// The publication area has two slots used by step parity: slot k % 2 is
+// written again at step k + 2, after every workgroup has passed the barrier
+// of step k + 1 and so has finished reading it.  Published values are
+// written and read with agent-scope atomic accesses (coherent across the
+// XCDs' L2 caches); every wave waits for its own stores to complete before
+// the workgroup signals arrival, so a workgroup that sees all arrival flags
+// reads complete data.
+#include "common.h"
+
+#include <algorithm>
+
+namespace rsdft {
+
+constexpr int kTrdThreads = 256;
+constexpr int kTrdWaves = kTrdThreads / 64;
+constexpr int kTrdMaxWG = 128;
+constexpr unsigned kTrdSpinLimit = 1u << 24;   // polls before a barrier gives up (seconds)
+
+__device__ __forceinline__ double trd_wave_sum(double x)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;   // the same value in every lane (addition commutes)
+}
+
+// sum over the workgroup, the same value in every thread; red: kTrdWaves doubles
+__device__ __forceinline__ double trd_block_sum(double x, double *red)
+{
+    x = trd_wave_sum(x);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < kTrdWaves; ++i) s += red[i];
+    return s;
+}
+
+// dlarfg on alpha = col[s+1], x = col[s+2:n]: v = (0.., 1, x / (alpha - beta)),
+// beta = -sign(alpha) |(alpha, x)|, tau = (beta - alpha) / beta; tau = 0 and
+// beta = alpha when x = 0.  Ends with a barrier.
+__device__ void trd_householder(const double *col, double *v, int n, int s, double *red,
+                                double &tau, double &beta)
+{
+    double ss = 0.0;
+    for (int i = s + 2 + (int)threadIdx.x; i < n; i += kTrdThreads) ss += col[i] * col[i];
+    const double xnorm2 = trd_block_sum(ss, red);
+    const double alpha = col[s + 1];
+    double scal = 0.0;
+    if (xnorm2 == 0.0) {
+        tau = 0.0;
+        beta = alpha;
+    } else {
+        beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
+        tau = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+    }
+    for (int i = threadIdx.x; i < n; i += kTrdThreads)
+        v[i] = i <= s ? 0.0 : (i == s + 1 ? 1.0 : col[i] * scal);
+    __syncthreads();
+}
+
+// Grid barrier on per-workgroup arrival flags (16 words apart): workgroup g
+// stores the step count in flags[16 g] once its stores have completed; the
+// first wave polls every flag in parallel.  A wait beyond the poll limit, or
+// an error raised by another workgroup, sets *err and makes every workgroup
+// leave; returns false then.
+__device__ bool trd_grid_barrier(unsigned *flags, unsigned target, int *err, int *stop)
+{
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's publication stores have completed
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x, nwg = gridDim.x;
+        if (lane == 0)
+            __hip_atomic_store(flags + 16 * blockIdx.x, target, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        unsigned polls = 0;
+        while (true) {
+            bool ok = true;
+            for (int g = lane; g < nwg; g += 64)
+                ok = ok && __hip_atomic_load(flags + 16 * g, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) >= target;
+            if (__all(ok)) break;
+            const bool bad =
+                __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+                ++polls > kTrdSpinLimit;
+            if (__any(bad)) {
+                if (lane == 0) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    *stop = 1;
+                }
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+    return *stop == 0;
+}
+
+__device__ __forceinline__ void trd_put(double *p, double x)
+{
+    __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double trd_get(const double *p)
+{
+    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A (n x n, symmetric, full storage) -> D, E, TAU, reflectors in R.
+// pub: 4n doubles; flags: 16 nwg words and err zero on entry.
+__global__ void __launch_bounds__(kTrdThreads)
+k_sytrd_coop(const double *__restrict__ A, int n, int cpw, double *__restrict__ R,
+             double *__restrict__ D, double *__restrict__ E, double *__restrict__ TAU,
+             double *pub, unsigned *flags, int *err)
+{
+    extern __shared__ double lds[];
+    __shared__ int stop;
+    const int nwg = gridDim.x, wg = blockIdx.x, tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    double *cols = lds;                     // [cpw][n] owned columns
+    double *v = cols + (size_t)cpw * n;     // [n] reflector of the current step
+    double *w = v + n;                      // [n] p, then w
+    double *col = w + n;                    // [n] column k+1, updated redundantly
+    double *red = col + n;                  // [kTrdWaves]
+
+    for (int c = 0; c < cpw; ++c) {
+        const int j = wg + c * nwg;
+        if (j >= n) break;
+        for (int i = tid; i < n; i += kTrdThreads) cols[(size_t)c * n + i] = A[(size_t)j * n + i];
+    }
+    for (int i = tid; i < n; i += kTrdThreads) col[i] = A[i];
+    if (tid == 0) stop = 0;
+    __syncthreads();
+    double tau, beta;
+    trd_householder(col, v, n, 0, red, tau, beta);
+    if (wg == 0) {
+        for (int i = 2 + tid; i < n; i += kTrdThreads) R[i] = v[i];
+        if (tid == 0) {
+            D[0] = col[0];
+            E[0] = beta;
+            TAU[0] = tau;
+        }
+    }
+    for (int k = 0; k <= n - 3; ++k) {
+        double *pp = pub + (size_t)(k & 1) * 2 * n, *pc = pp + n;
+        // a. p_j = tau v^T A(:, j) for the owned columns j > k, one wave per column
+        for (int c = wave; c < cpw; c += kTrdWaves) {
+            const int j = wg + c * nwg;
+            if (j >= n) break;
+            if (j <= k) continue;
+            const double *a = cols + (size_t)c * n;
+            double s = 0.0;
+            for (int i = k + 1 + lane; i < n; i += 64) s += a[i] * v[i];
+            s = tau * trd_wave_sum(s);
+            if (lane == 0) trd_put(pp + j, s);
+        }
+        if ((k + 1) % nwg == wg) {
+            const double *a = cols + (size_t)((k + 1) / nwg) * n;
+            for (int i = k + 1 + tid; i < n; i += kTrdThreads) trd_put(pc + i, a[i]);
+        }
+        if (!trd_grid_barrier(flags, (unsigned)(k + 1), err, &stop)) return;
+        // b. the whole p and the column k+1; p^T v
+        double t = 0.0;
+        for (int i = k + 1 + tid; i < n; i += kTrdThreads) {
+            const double pi = trd_get(pp + i);
+            w[i] = pi;
+            col[i] = trd_get(pc + i);
+            t += pi * v[i];
+        }
+        const double alpha = -0.5 * tau * trd_block_sum(t, red);   // ends with a barrier
+        for (int i = k + 1 + tid; i < n; i += kTrdThreads) w[i] += alpha * v[i];
+        __syncthreads();
+        for (int c = 0; c < cpw; ++c) {
+            const int j = wg + c * nwg;
+            if (j >= n) break;
+            if (j <= k) continue;
+            const double wj = w[j], vj = v[j];
+            double *a = cols + (size_t)c * n;
+            for (int i = k + 1 + tid; i < n; i += kTrdThreads) a[i] -= v[i] * wj + w[i] * vj;
+        }
+        {
+            const double wj = w[k + 1], vj = v[k + 1];
+            for (int i = k + 1 + tid; i < n; i += kTrdThreads) col[i] -= v[i] * wj + w[i] * vj;
+        }
+        __syncthreads();
+        trd_householder(col, v, n, k + 1, red, tau, beta);
+        if ((k + 1) % nwg == wg) {
+            for (int i = k + 3 + tid; i < n; i += kTrdThreads) R[(size_t)(k + 1) * n + i] = v[i];
+            if (tid == 0) {
+                D[k + 1] = col[k + 1];
+                E[k + 1] = beta;
+                TAU[k + 1] = tau;
+            }
+        }
+    }
+    if ((n - 1) % nwg == wg && tid == 0) D[n - 1] = cols[(size_t)((n - 1) / nwg) * n + n - 1];
+}
+
+// Z <- Q Z = H(0) H(1) ... H(n-2) Z (dormtr('L', 'L', 'N')): the columns of Z
+// are independent, so each wave carries kQCols of them in registers (NPL
+// values per lane, n <= 64 NPL) through all reflectors, the next reflector
+// loaded while the current one is applied.
+constexpr int kQCols = 2;
+template <int NPL>
+__global__ void __launch_bounds__(256)
+k_apply_q(const double *__restrict__ R, const double *__restrict__ TAU, int n, double *Z)
+{
+    const int wave = (int)((blockIdx.x * 256u + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+    const int c0 = wave * kQCols;
+    if (c0 >= n) return;
+    double z[kQCols][NPL];
+#pragma unroll
+    for (int q = 0; q < kQCols; ++q)
+#pragma unroll
+        for (int m = 0; m < NPL; ++m) {
+            const int r = lane + 64 * m;
+            z[q][m] = (c0 + q < n && r < n) ? Z[(size_t)(c0 + q) * n + r] : 0.0;
+        }
+    auto load_v = [&](int i, double (&v)[NPL]) {
+#pragma unroll
+        for (int m = 0; m < NPL; ++m) {
+            const int r = lane + 64 * m;
+            v[m] = r == i + 1 ? 1.0 : (r > i + 1 && r < n ? R[(size_t)i * n + r] : 0.0);
+        }
+    };
+    double vc[NPL], vn[NPL];
+    load_v(n - 2, vc);
+    for (int i = n - 2; i >= 0; --i) {
+        if (i > 0) load_v(i - 1, vn);
+        const double t = TAU[i];
+        double s[kQCols];
+#pragma unroll
+        for (int q = 0; q < kQCols; ++q) {
+            double a = 0.0;
+#pragma unroll
+            for (int m = 0; m < NPL; ++m) a += vc[m] * z[q][m];
+            s[q] = t * trd_wave_sum(a);
+        }
+#pragma unroll
+        for (int q = 0; q < kQCols; ++q)
+#pragma unroll
+            for (int m = 0; m < NPL; ++m) z[q][m] -= s[q] * vc[m];
+#pragma unroll
+        for (int m = 0; m < NPL; ++m) vc[m] = vn[m];
+    }
+#pragma unroll
+    for (int q = 0; q < kQCols; ++q)
+#pragma unroll
+        for (int m = 0; m < NPL; ++m) {
+            const int r = lane + 64 * m;
+            if (c0 + q < n && r < n) Z[(size_t)(c0 + q) * n + r] = z[q][m];
+        }
+}
+
+__global__ void k_trd_clear(unsigned *flags, int nflags, int *err)
+{
+    for (int i = threadIdx.x; i < nflags; i += blockDim.x) flags[i] = 0u;
+    if (threadIdx.x == 0) *err = 0;
+}
+
+// flag = -1 when a grid barrier gave up, else the dstedc info
+__global__ void k_trd_flag(const int *err, const int *info, int *flag)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (*err != 0) *flag = -1;
+        else if (*info != 0) *flag = *info;
+    }
+}
+
+// workgroups and LDS bytes of the cooperative reduction (about 4 columns per
+// workgroup), false if n does not fit
+static bool trd_shape(int n, int *nwg, int *cpw, size_t *lds)
+{
+    Context &c = ctx();
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess)
+        return false;
+    if (n < 3) return false;
+    const long budget = (long)(c.max_lds / sizeof(double)) - 3L * n - kTrdWaves - 64;
+    if (budget < n) return false;
+    const int cmax = (int)std::min<long>(budget / n, 1 << 20);
+    int g = std::min(std::min(kTrdMaxWG, cus), (n + 3) / 4);
+    g = std::max(g, (n + cmax - 1) / cmax);
+    if (g > std::min(kTrdMaxWG, cus)) return false;
+    *nwg = g;
+    *cpw = (n + g - 1) / g;
+    *lds = sizeof(double) * ((size_t)(*cpw) * n + 3 * (size_t)n + kTrdWaves);
+    return *lds <= c.max_lds;
+}
+
+// Eigenpairs of the symmetric H (n x n, full storage): eigenvalues ascending
+// in D, eigenvectors in V.  einfo[0] receives the dstedc info and einfo[3] a
+// nonzero flag on failure.  Returns kOK with *done = 0 when the matrix is too
+// large for the LDS-resident reduction (nothing launched then).
+int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *done)
+{
+    Context &c = ctx();
+    *done = 0;
+    int nwg = 0, cpw = 0;
+    size_t lds = 0;
+    if (!trd_shape(n, &nwg, &cpw, &lds)) return kOK;
+    const size_t nn = (size_t)n * n;
+    const int nflags = 16 * nwg;
+    RS_TRY(c.evwork.ensure(sizeof(double) * (nn + 6 * (size_t)n) + sizeof(int) * (nflags + 4)));
+    double *R = c.evwork.as<double>(), *E = R + nn, *TAU = E + n, *pub = TAU + n;
+    unsigned *flags = reinterpret_cast<unsigned *>(pub + 4 * (size_t)n);
+    int *err = reinterpret_cast<int *>(flags + nflags);
+    if (lds > 64 * 1024)
+        RS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_sytrd_coop),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_trd_clear, dim3(1), dim3(256), 0, c.stream, flags, nflags, err);
+    RS_HIP(hipGetLastError());
+    void *args[] = {&H, &n, &cpw, &R, &D, &E, &TAU, &pub, &flags, &err};
+    RS_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_sytrd_coop), dim3(nwg),
+                                      dim3(kTrdThreads), args, (unsigned)lds, c.stream));
+    RS_BLAS(rocsolver_dstedc(c.blas, rocblas_evect_tridiagonal, n, D, E, V, n, einfo));
+    const int waves = (n + kQCols - 1) / kQCols;
+    const dim3 qgrid((waves + 3) / 4), qblk(256);
+    if (n <= 64) hipLaunchKernelGGL((k_apply_q<1>), qgrid, qblk, 0, c.stream, R, TAU, n, V);
+    else if (n <= 128) hipLaunchKernelGGL((k_apply_q<2>), qgrid, qblk, 0, c.stream, R, TAU, n, V);
+    else if (n <= 256) hipLaunchKernelGGL((k_apply_q<4>), qgrid, qblk, 0, c.stream, R, TAU, n, V);
+    else if (n <= 512) hipLaunchKernelGGL((k_apply_q<8>), qgrid, qblk, 0, c.stream, R, TAU, n, V);
+    else
+        RS_BLAS(rocsolver_dormtr(c.blas, rocblas_side_left, rocblas_fill_lower,
+                                 rocblas_operation_none, n, n, R, n, TAU, V, n));
+    hipLaunchKernelGGL(k_trd_flag, dim3(1), dim3(64), 0, c.stream, err, einfo, einfo + 3);
+    RS_HIP(hipGetLastError());
+    *done = 1;
+    return kOK;
+}
+
+}  // namespace rsdft

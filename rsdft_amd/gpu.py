@@ -32,12 +32,13 @@ EXPORTS = (
     "rsdft_gpu_init", "rsdft_gpu_set_vloc", "rsdft_gpu_set_projectors",
     "rsdft_gpu_set_projector_derivatives", "rsdft_gpu_set_precision",
     "rsdft_gpu_set_solver_params", "rsdft_gpu_set_precond_mode", "rsdft_gpu_set_eigensolver",
+    "rsdft_gpu_set_sweep_mode",
     "rsdft_gpu_hamiltonian", "rsdft_gpu_op_kinetic",
     "rsdft_gpu_op_localpot", "rsdft_gpu_op_nonlocal", "rsdft_gpu_wf_attach",
     "rsdft_gpu_wf_sync", "rsdft_gpu_wf_detach", "rsdft_gpu_conjugate_gradient",
     "rsdft_gpu_gram_schmidt", "rsdft_gpu_subspace_diag", "rsdft_gpu_sweep",
     "rsdft_gpu_calc_density", "rsdft_gpu_force_ps_nloc2", "rsdft_gpu_band_energy_grad",
-    "rsdft_gpu_test_hpsi", "rsdft_gpu_get_timers", "rsdft_gpu_reset_timers",
+    "rsdft_gpu_test_hpsi", "rsdft_gpu_test_eigh", "rsdft_gpu_get_timers", "rsdft_gpu_reset_timers",
     "rsdft_gpu_finalize", "rsdft_gpu_last_error",
 )
 
@@ -67,6 +68,7 @@ def load_library(path: str = LIB_PATH):
         "rsdft_gpu_set_solver_params": [I, I],
         "rsdft_gpu_set_precond_mode": [I],
         "rsdft_gpu_set_eigensolver": [I],
+        "rsdft_gpu_set_sweep_mode": [I],
         "rsdft_gpu_hamiltonian": [I, I, P, P, I, I, I, I],
         "rsdft_gpu_op_kinetic": [I, P, P, I, I, I, I],
         "rsdft_gpu_op_localpot": [I, I, I, P, P],
@@ -81,6 +83,7 @@ def load_library(path: str = LIB_PATH):
         "rsdft_gpu_calc_density": [P, I, I, P, P],
         "rsdft_gpu_force_ps_nloc2": [P, I, I, P, P],
         "rsdft_gpu_band_energy_grad": [P, I, I, P, P, P, P],
+        "rsdft_gpu_test_eigh": [P, I, I, P, P, P],
         "rsdft_gpu_test_hpsi": [I, P],
         "rsdft_gpu_get_timers": [P, I],
         "rsdft_gpu_reset_timers": [],
@@ -146,8 +149,14 @@ class RsdftGpu:
         self._call("rsdft_gpu_set_precond_mode", int(mode))
 
     def set_eigensolver(self, mode: int):
-        """1: refinement when Hsub is nearly diagonal (default), 0: always dsyevd."""
+        """1: refinement when Hsub is nearly diagonal, else mode 2 (default);
+        2: LDS-resident tridiagonal reduction + dstedc + dormtr; 0: always dsyevd."""
         self._call("rsdft_gpu_set_eigensolver", int(mode))
+
+    def set_sweep_mode(self, mode: int):
+        """1: fused Gram-Schmidt + Rayleigh-Ritz from CG's H x (default),
+        0: gram_schmidt then subspace_diag as two steps."""
+        self._call("rsdft_gpu_set_sweep_mode", int(mode))
 
     def set_vloc(self, vloc):
         v = _f64(vloc)
@@ -263,6 +272,17 @@ class RsdftGpu:
         return float(E[0]), dV, dR.reshape(self.natom, 3)
 
     # -- measurement ---------------------------------------------------------------------
+    def test_eigh(self, H, mode: int = 2, vectors: bool = True):
+        """(w, V, ms): eigenpairs of the symmetric matrix H by eigensolver `mode`."""
+        A = np.asfortranarray(_f64(H))
+        n = A.shape[0]
+        w = np.zeros(n)
+        V = np.zeros((n, n), order="F") if vectors else None
+        ms = np.zeros(1)
+        self._call("rsdft_gpu_test_eigh", _p(A), n, int(mode), _p(w),
+                   _p(V) if vectors else None, _p(ms))
+        return w, V, float(ms[0])
+
     def test_hpsi(self, nrep: int = 10):
         """ms per all-band application: hamiltonian, kinetic, localpot, nonlocal,
         stencil kernel of the hamiltonian, projection kernel, nonlocal term

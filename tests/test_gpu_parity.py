@@ -60,10 +60,18 @@ def test_gram_schmidt_fp64(dev, oracle_si8, si8):
     assert rel(q, qo) < 1e-11
 
 
-def test_subspace_diag_fp64(dev, oracle_si8, si8):
-    print("case: subspace_diag fp64 si8", flush=True)
+@pytest.mark.parametrize("eig_mode", [1, 0, 2])
+def test_subspace_diag_fp64(dev, oracle_si8, si8, eig_mode):
+    """eig_mode 1 (refinement, here falling back to the tridiagonal reduction:
+    Hsub of random vectors is far from diagonal), 0 rocsolver_dsyevd, 2 the
+    LDS-resident tridiagonal reduction + dstedc + dormtr."""
+    print(f"case: subspace_diag fp64 si8 eigensolver mode {eig_mode}", flush=True)
     q = oracle_si8.gram_schmidt(si8.psi0())
-    x, e = dev.subspace_diag(q)
+    dev.set_eigensolver(eig_mode)
+    try:
+        x, e = dev.subspace_diag(q)
+    finally:
+        dev.set_eigensolver(1)
     xo, eo, _ = oracle_si8.subspace_diag(q)
     assert np.max(np.abs(e - eo)) < 1e-10
     # Ritz vectors agree up to sign
@@ -85,19 +93,26 @@ def test_conjugate_gradient_fp64(dev, oracle_si8, si8, pc_mode):
     assert rel(x, xo) < 1e-9
 
 
-@pytest.mark.parametrize("pc_mode", [1, 0])
-def test_four_sweeps_fp64(dev, oracle_si8, si8, pc_mode):
-    print(f"case: 4 sweeps fp64 si8 precond mode {pc_mode}", flush=True)
+@pytest.mark.parametrize("pc_mode,sweep_mode", [(1, 1), (0, 1), (1, 0)])
+def test_four_sweeps_fp64(dev, oracle_si8, si8, pc_mode, sweep_mode):
+    print(f"case: 4 sweeps fp64 si8 precond mode {pc_mode} sweep mode {sweep_mode}", flush=True)
     dev.set_precond_mode(pc_mode)
+    dev.set_sweep_mode(sweep_mode)
     q = oracle_si8.gram_schmidt(si8.psi0())
     x, eo = q, None
     xg = q.copy()
-    for it in range(4):
-        x, eo, ro = oracle_si8.sweep(x)
-        xg, eg, rg = dev.sweep(xg, 1)
-        print(f"  sweep {it}: max|de| = {np.max(np.abs(eg - eo)):.3e}", flush=True)
-        assert np.max(np.abs(eg - eo)) < 1e-10
-    dev.set_precond_mode(1)
+    try:
+        for it in range(4):
+            x, eo, ro = oracle_si8.sweep(x)
+            xg, eg, rg = dev.sweep(xg, 1)
+            print(f"  sweep {it}: max|de| = {np.max(np.abs(eg - eo)):.3e}", flush=True)
+            assert np.max(np.abs(eg - eo)) < 1e-10
+        # the same orthonormal span: the overlap dV X_gpu X_oracle^T is orthogonal
+        m = oracle_si8.dV * (xg @ x.T)
+        assert np.max(np.abs(m.T @ m - np.eye(m.shape[0]))) < 1e-9
+    finally:
+        dev.set_precond_mode(1)
+        dev.set_sweep_mode(1)
 
 
 def test_density_and_force_fp64(dev, oracle_si8, si8):
@@ -126,8 +141,12 @@ def test_refinement_eigensolver_matches_dsyevd(dev, oracle_si8, si8):
     used = dev.timers()["eig_refined"]
     dev.set_eigensolver(0)
     x0, e0 = dev.subspace_diag(q)
+    dev.set_eigensolver(2)
+    x2, e2 = dev.subspace_diag(q)
     dev.set_eigensolver(1)
     assert used == 1
-    assert np.max(np.abs(e1 - eo)) < 1e-10 and np.max(np.abs(e0 - eo)) < 1e-10
-    s = np.sign(np.sum(x1 * xo, axis=1))
-    assert rel(x1 * s[:, None], xo) < 1e-8
+    for e in (e1, e0, e2):
+        assert np.max(np.abs(e - eo)) < 1e-10
+    for x in (x1, x2):
+        s = np.sign(np.sum(x * xo, axis=1))
+        assert rel(x * s[:, None], xo) < 1e-8
