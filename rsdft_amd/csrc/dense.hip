@@ -499,6 +499,7 @@ static int refine_eigh(const double *H, int n, double *V, double *D, int *ok)
 }
 
 // C(nb x nb) = alpha * A^T B, A and B (ML x nb) column-major, result FP64
+// (rocblas_dsyrk for A == B measured ~10x slower than dgemm at 480 x 373,248)
 template <class T>
 static int gemm_tn(const T *A, const T *B, int nb, double alpha, double *C)
 {

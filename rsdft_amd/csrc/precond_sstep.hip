@@ -25,6 +25,7 @@
 #include "common.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace rsdft {
 
@@ -73,12 +74,20 @@ __host__ __device__ constexpr int qout(int l, int NLEV) {
 // x/y neighbours of plane s-l taken from LDS, where level l-1 left it at the
 // end of step s-1.  Every level's newest plane is staged into one of two LDS
 // buffers at the end of the step, so one barrier per plane suffices.
-// FP64 keeps 768 threads so the compiler may use up to 168 registers
-template <class T>
-constexpr int sstep_max_threads() { return sizeof(T) == 8 ? 768 : 1024; }
+// workgroup size limit: fewer threads leave more registers per thread
+// (FP64 768 threads: up to 168 registers; five levels need more)
+template <class T, int NLEV>
+constexpr int sstep_max_threads()
+{
+    return sizeof(T) == 8 ? (NLEV >= 5 ? 512 : 768) : (NLEV >= 5 ? 768 : 1024);
+}
+static int sstep_max_threads_rt(size_t elem, int nlev)
+{
+    return elem == 8 ? (nlev >= 5 ? 512 : 768) : (nlev >= 5 ? 768 : 1024);
+}
 
 template <class T, int PX, int NLEV, int MODE>
-__global__ void __launch_bounds__(sstep_max_threads<T>())
+__global__ void __launch_bounds__((sstep_max_threads<T, NLEV>()))
 k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2, int N3,
            int ty, int tz, int tpr, double c0, double c1, double c2, double c3, double s1,
            int nb, const int *__restrict__ active, const double *__restrict__ dcoef,
@@ -87,7 +96,7 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
     constexpr int NG = NLEV * (NLEV + 1) / 2 + NLEV;  // Gram entries
     constexpr int QTOT = qoff(NLEV, NLEV) + 1;
     constexpr int PF = kSstepPF;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T *Lbase = reinterpret_cast<T *>(smem_raw);
     __shared__ double red[16][MODE == 0 ? NG : 1];
 
@@ -101,16 +110,38 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
     const int y0 = (tile % nty) * ty;
     const int z0 = (tile / nty) * tz;
     const int rows = ty + 2 * NLEV;
-    const int W = N1 + 2;
-    const int LS = rows * W;              // one level plane in LDS
+    // LDS row: [pad, x = -1, x = 0 .. N1-1, x = N1, pad]; a thread's PX
+    // adjacent points start at an even index, so they move as one vector
+    const int W = N1 + 4;
+    const int LS = rows * W;                 // one level plane in LDS
     const int tid = threadIdx.x;
     const int r = tid / tpr;                 // 0 .. rows-1
-    const int cx0 = tid - r * tpr;
+    const int x0 = (tid - r * tpr) * PX;
     const int gy = wrapz(y0 - NLEV + r, N2);
     const long plane = (long)N1 * N2;
     const T *src = G + (long)band * ML;
-    const long rowoff = (long)gy * N1 + cx0;
+    const long rowoff = (long)gy * N1 + x0;
     const bool interior = r >= NLEV && r < NLEV + ty;
+    typedef T VP __attribute__((ext_vector_type(PX)));
+
+    // this thread's LDS positions (x0 - 1 of its row, and the rows above and
+    // below) in every level plane of both buffers: each access below is an
+    // immediate offset from one of them
+    // (with five levels the row pointers are formed per step instead, to
+    // stay within the register budget)
+    constexpr bool kRowPtrs = NLEV <= 4;
+    T *pc[2][NLEV], *pm[2][kRowPtrs ? NLEV : 1], *pp[2][kRowPtrs ? NLEV : 1];
+#pragma unroll
+    for (int bf = 0; bf < 2; ++bf)
+#pragma unroll
+        for (int l = 0; l < NLEV; ++l) {
+            T *b0 = Lbase + (long)(bf * NLEV + l) * LS + r * W + 1 + x0;
+            pc[bf][l] = b0;
+            if constexpr (kRowPtrs) {
+                pm[bf][l] = b0 - W;
+                pp[bf][l] = b0 + W;
+            }
+        }
 
     double dk[NLEV > 0 ? NLEV : 1];
     if (MODE >= 1) {
@@ -135,39 +166,52 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
 #pragma unroll
         for (int k = 0; k < D0 - 1; ++k) {
             const int zz = s0 - (D0 - 1 - PF) + k;
-            const T *p = src + (long)wrapz(zz, N3) * plane + rowoff;
+            const VP v = *reinterpret_cast<const VP *>(src + (long)wrapz(zz, N3) * plane + rowoff);
 #pragma unroll
-            for (int px = 0; px < PX; ++px) q[qoff(0, NLEV) + k][px] = p[px * tpr];
+            for (int px = 0; px < PX; ++px) q[qoff(0, NLEV) + k][px] = v[px];
         }
     }
 
     const T tc0 = (T)c0, tc1 = (T)c1, tc2 = (T)c2, tc3 = (T)c3, ts1 = (T)s1;
-    for (int s = s0; s < s1e; ++s) {
-        T *Lcur = Lbase + (long)((s - s0) & 1) * NLEV * LS;       // written at the end of this step
-        const T *Lprev = Lbase + (long)(((s - s0) & 1) ^ 1) * NLEV * LS;   // left by step s-1
+    // one plane step; CUR is the LDS buffer written at its end ((s - s0) & 1),
+    // the other one holds what step s-1 left
+    auto step = [&](auto cur_tag, int s) {
+        constexpr int CUR = decltype(cur_tag)::value, PREV = CUR ^ 1;
         {   // request plane s + PF of g
             int zn = s + PF;
             while (zn < 0) zn += N3;
             while (zn >= N3) zn -= N3;
-            const T *p = src + (long)zn * plane + rowoff;
+            const VP v = *reinterpret_cast<const VP *>(src + (long)zn * plane + rowoff);
 #pragma unroll
-            for (int px = 0; px < PX; ++px) q[qoff(0, NLEV) + qdepth(0, NLEV) - 1][px] = p[px * tpr];
+            for (int px = 0; px < PX; ++px) q[qoff(0, NLEV) + qdepth(0, NLEV) - 1][px] = v[px];
         }
 #pragma unroll
         for (int l = 0; l < NLEV; ++l) {
             // level l+1 at plane s-l-1 on rows [l+1, rows-l-1): level l at planes
-            // s-l-2 .. s-l (slots qnew-2 .. qnew), x/y of plane s-l-1 from Lprev
-            const T *Lp = Lprev + (long)l * LS;
+            // s-l-2 .. s-l (slots qnew-2 .. qnew); the neighbours across the
+            // chunk edges and rows of plane s-l-1 from the previous buffer
             const int qc = qnew(l, NLEV) - 1;
             const int qn = qnew(l + 1, NLEV);
             if (r >= l + 1 && r < rows - l - 1) {
+                const T *c = pc[PREV][l];
+                const T *cm, *cp;
+                if constexpr (kRowPtrs) {
+                    cm = pm[PREV][l];
+                    cp = pp[PREV][l];
+                } else {
+                    cm = c - W;
+                    cp = c + W;
+                }
+                const VP ym = *reinterpret_cast<const VP *>(cm + 1);
+                const VP yp = *reinterpret_cast<const VP *>(cp + 1);
+                const T xl = c[0], xr = c[PX + 1];
 #pragma unroll
                 for (int px = 0; px < PX; ++px) {
-                    const int x = cx0 + px * tpr;
-                    const T *lr = Lp + r * W + 1 + x;
                     const T cen = q[qc][px];
+                    const T left = px == 0 ? xl : q[qc][px - 1];
+                    const T right = px == PX - 1 ? xr : q[qc][px + 1];
                     T a = tc0 * cen;
-                    a = a + tc1 * (lr[1] + lr[-1]) + tc2 * (lr[W] + lr[-W]) +
+                    a = a + tc1 * (right + left) + tc2 * (yp[px] + ym[px]) +
                         tc3 * (q[qc + 1][px] + q[qc - 1][px]);
                     T v;
                     if (l == 0) {
@@ -187,54 +231,50 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
         const int zo = s - NLEV;
         if (interior && zo >= z0) {
             if (MODE == 0) {
-                T vv[NLEV + 1][PX];
-#pragma unroll
-                for (int l = 0; l <= NLEV; ++l)
-#pragma unroll
-                    for (int px = 0; px < PX; ++px) vv[l][px] = q[qout(l, NLEV)][px];
 #pragma unroll
                 for (int px = 0; px < PX; ++px) {
+                    T vv[NLEV + 1];
+#pragma unroll
+                    for (int l = 0; l <= NLEV; ++l) vv[l] = q[qout(l, NLEV)][px];
                     int e = 0;
 #pragma unroll
                     for (int j = 1; j <= NLEV; ++j)
 #pragma unroll
                         for (int k = j; k <= NLEV; ++k)
-                            gacc[e++] += (double)vv[j][px] * (double)vv[k][px];
+                            gacc[e++] += (double)vv[j] * (double)vv[k];
 #pragma unroll
                     for (int k = 0; k < NLEV; ++k)
-                        gacc[e++] += (double)vv[0][px] * (double)vv[k][px];
+                        gacc[e++] += (double)vv[0] * (double)vv[k];
                 }
             } else {
-                T *dst = PG + (long)band * ML + (long)zo * plane + rowoff;
+                VP *dst = reinterpret_cast<VP *>(PG + (long)band * ML + (long)zo * plane + rowoff);
+                VP old;
+                if (MODE == 2 && beta) old = *dst;
+                VP outv;
 #pragma unroll
                 for (int px = 0; px < PX; ++px) {
                     double x = (double)q[qout(0, NLEV)][px];
 #pragma unroll
                     for (int k = 0; k < NLEV; ++k) x += dk[k] * (double)q[qout(k + 1, NLEV)][px];
-                    if (MODE == 2) {
-                        // P = Pg + beta P in the working precision, as k_cg_newp
-                        T p = (T)x;
-                        if (beta) p = p + (T)bk * dst[px * tpr];
-                        dst[px * tpr] = p;
-                    } else {
-                        dst[px * tpr] = (T)x;
-                    }
+                    // MODE 2: P = Pg + beta P in the working precision, as k_cg_newp
+                    T pv = (T)x;
+                    if (MODE == 2 && beta) pv = pv + (T)bk * old[px];
+                    outv[px] = pv;
                 }
+                *dst = outv;
             }
         }
         // stage the newest plane of levels 0 .. NLEV-1 for the next step
 #pragma unroll
         for (int l = 0; l < NLEV; ++l) {
             if (r >= l && r < rows - l) {
-                T *Lp = Lcur + (long)l * LS;
+                T *sc = pc[CUR][l] + 1;
+                VP v;
 #pragma unroll
-                for (int px = 0; px < PX; ++px) {
-                    const int x = cx0 + px * tpr;
-                    const T v = q[qnew(l, NLEV)][px];
-                    Lp[r * W + 1 + x] = v;
-                    if (x == 0) Lp[r * W + 1 + N1] = v;
-                    if (x == N1 - 1) Lp[r * W] = v;
-                }
+                for (int px = 0; px < PX; ++px) v[px] = q[qnew(l, NLEV)][px];
+                *reinterpret_cast<VP *>(sc) = v;
+                if (x0 == 0) sc[N1] = v[0];                       // x = N1
+                if (x0 + PX == N1) sc[-x0 - 1] = v[PX - 1];       // x = -1
             }
         }
         __syncthreads();
@@ -245,6 +285,10 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
             for (int k = 0; k < qdepth(l, NLEV) - 1; ++k)
 #pragma unroll
                 for (int px = 0; px < PX; ++px) q[qoff(l, NLEV) + k][px] = q[qoff(l, NLEV) + k + 1][px];
+    };
+    for (int s = s0; s < s1e; s += 2) {
+        step(std::integral_constant<int, 0>{}, s);
+        if (s + 1 < s1e) step(std::integral_constant<int, 1>{}, s + 1);
     }
     if (MODE == 0) {
         const int lane = tid & 63, wave = tid >> 6, nw = (blockDim.x + 63) >> 6;
@@ -356,7 +400,7 @@ static SstepGeom sstep_geom(const GridDesc &g, int nlev, int nlds, int px, int m
         if (g.N2 % ty) continue;
         const int thr = tpr * (ty + 2 * nlev);
         if (thr > maxthr) break;
-        const size_t lds = 2 * (size_t)nlds * (ty + 2 * nlev) * (g.N1 + 2) * elem;
+        const size_t lds = 2 * (size_t)nlds * (ty + 2 * nlev) * (g.N1 + 4) * elem;
         if (lds > maxlds) break;
         G.ty = ty;
         G.threads = thr;
@@ -408,8 +452,8 @@ static int sstep_coeffs(const T *g, int nb, const int *active, double *part, dou
     const GridDesc &gd = c.g;
     constexpr int PX = 2;
     const SstepConst k = sstep_const(gd);
-    const SstepGeom G = sstep_geom(gd, M + 1, M + 1, PX, sstep_max_threads<T>(), sizeof(T),
-                                   c.max_lds);
+    const SstepGeom G = sstep_geom(gd, M + 1, M + 1, PX, sstep_max_threads<T, M + 1>(),
+                                   sizeof(T), c.max_lds);
     if (G.ty == 0) return set_error(kErrArg, "no s-step tiling");
     const size_t lds = G.lds;
     const long nblk = (long)G.tiles * nb;
@@ -434,7 +478,7 @@ static int sstep_apply(const T *g, T *out, int nb, const int *active, const doub
     const GridDesc &gd = c.g;
     constexpr int PX = 2;
     const SstepConst k = sstep_const(gd);
-    const SstepGeom G = sstep_geom(gd, M, M, PX, sstep_max_threads<T>(), sizeof(T), c.max_lds);
+    const SstepGeom G = sstep_geom(gd, M, M, PX, sstep_max_threads<T, M>(), sizeof(T), c.max_lds);
     if (G.ty == 0) return set_error(kErrArg, "no s-step tiling");
     const size_t lds = G.lds;
     const long nblk = (long)G.tiles * nb;
@@ -459,8 +503,10 @@ size_t precond_sstep_part_elems(int nb)
     Context &c = ctx();
     size_t mx = 0;
     for (int m = 1; m <= 4; ++m) {
-        const SstepGeom G = sstep_geom(c.g, m + 1, m + 1, 2, 768, 8, c.max_lds);
-        const SstepGeom G4 = sstep_geom(c.g, m + 1, m + 1, 2, 1024, 4, c.max_lds);
+        const SstepGeom G = sstep_geom(c.g, m + 1, m + 1, 2, sstep_max_threads_rt(8, m + 1), 8,
+                                       c.max_lds);
+        const SstepGeom G4 = sstep_geom(c.g, m + 1, m + 1, 2, sstep_max_threads_rt(4, m + 1), 4,
+                                        c.max_lds);
         const size_t t = (size_t)std::max(G.tiles, G4.tiles);
         mx = std::max(mx, t * (size_t)((m + 1) * (m + 2) / 2 + m + 1) * nb);
     }
