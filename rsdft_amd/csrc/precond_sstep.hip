@@ -40,23 +40,27 @@ __device__ __forceinline__ long xcd_map(long L, long total) {
     return (L & 7) * per + (L >> 3);
 }
 
-// Queue depth of level l for NLEV levels above g: three planes for the next
-// level's stencil, and back to plane s - NLEV where all levels are combined;
-// level 0 (g) also holds the PF planes requested ahead.
+// Register rings of the levels.  Level l keeps its last rdepth(l) planes:
+// three for the next level's stencil and back to plane s - NLEV where all
+// levels are combined; level 0 (g) also holds the PF planes requested ahead.
+// Every depth divides kRing and the z loop is unrolled kRing times, so plane
+// z of level l sits in slot roff(l) + (z - s0) mod rdepth(l), a compile-time
+// index in every unrolled step: no register moves between planes.
 constexpr int kSstepPF = 2;
-__host__ __device__ constexpr int qdepth(int l, int NLEV) {
-    return l == NLEV ? 1 : ((NLEV - l + 1) > 3 ? (NLEV - l + 1) : 3) + (l == 0 ? kSstepPF : 0);
+constexpr int kRing = 8;
+__host__ __device__ constexpr int rneed(int l, int NLEV) {
+    return l == NLEV ? 1
+                     : ((NLEV - l + 1) > 3 ? (NLEV - l + 1) : 3) + (l == 0 ? kSstepPF : 0);
 }
-__host__ __device__ constexpr int qoff(int l, int NLEV) {
-    return l == 0 ? 0 : qoff(l - 1, NLEV) + qdepth(l - 1, NLEV);
+__host__ __device__ constexpr int rdepth(int l, int NLEV) {
+    return rneed(l, NLEV) == 1 ? 1 : (rneed(l, NLEV) <= 2 ? 2 : (rneed(l, NLEV) <= 4 ? 4 : 8));
 }
-// slot of the newest computed plane of level l (level 0: the current plane s)
-__host__ __device__ constexpr int qnew(int l, int NLEV) {
-    return qoff(l, NLEV) + qdepth(l, NLEV) - 1 - (l == 0 ? kSstepPF : 0);
+__host__ __device__ constexpr int roff(int l, int NLEV) {
+    return l == 0 ? 0 : roff(l - 1, NLEV) + rdepth(l - 1, NLEV);
 }
-// slot of level l holding plane s - NLEV (the plane where all levels meet)
-__host__ __device__ constexpr int qout(int l, int NLEV) {
-    return qnew(l, NLEV) - (NLEV - l);
+// slot of level l for the plane d planes after s0 (d may be negative)
+__host__ __device__ constexpr int rslot(int l, int NLEV, int d) {
+    return roff(l, NLEV) + ((d % rdepth(l, NLEV)) + rdepth(l, NLEV)) % rdepth(l, NLEV);
 }
 
 // MODE 0: Gram pass, NLEV = m + 1, part[(e*nb+b)*tiles + tile] over the
@@ -94,7 +98,7 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
            double *__restrict__ part, int tiles, const double *__restrict__ beta)
 {
     constexpr int NG = NLEV * (NLEV + 1) / 2 + NLEV;  // Gram entries
-    constexpr int QTOT = qoff(NLEV, NLEV) + 1;
+    constexpr int QTOT = roff(NLEV, NLEV) + rdepth(NLEV, NLEV);
     constexpr int PF = kSstepPF;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T *Lbase = reinterpret_cast<T *>(smem_raw);
@@ -162,36 +166,38 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
     const int s0 = z0 - NLEV, s1e = z0 + tz + NLEV;
     // level 0: planes s0 - (depth-1-PF) .. s0 + PF - 1 (the prologue of the queue)
     {
-        constexpr int D0 = qdepth(0, NLEV);
+        constexpr int D0 = rneed(0, NLEV);
 #pragma unroll
         for (int k = 0; k < D0 - 1; ++k) {
-            const int zz = s0 - (D0 - 1 - PF) + k;
-            const VP v = *reinterpret_cast<const VP *>(src + (long)wrapz(zz, N3) * plane + rowoff);
+            const int d = -(D0 - 1 - PF) + k;   // plane s0 + d
+            const VP v = *reinterpret_cast<const VP *>(src + (long)wrapz(s0 + d, N3) * plane + rowoff);
 #pragma unroll
-            for (int px = 0; px < PX; ++px) q[qoff(0, NLEV) + k][px] = v[px];
+            for (int px = 0; px < PX; ++px) q[rslot(0, NLEV, d)][px] = v[px];
         }
     }
 
     const T tc0 = (T)c0, tc1 = (T)c1, tc2 = (T)c2, tc3 = (T)c3, ts1 = (T)s1;
-    // one plane step; CUR is the LDS buffer written at its end ((s - s0) & 1),
-    // the other one holds what step s-1 left
-    auto step = [&](auto cur_tag, int s) {
-        constexpr int CUR = decltype(cur_tag)::value, PREV = CUR ^ 1;
+    // one plane step, s = s0 + U (mod kRing); CUR is the LDS buffer written at
+    // its end ((s - s0) & 1), the other one holds what step s-1 left
+    auto step = [&](auto u_tag, int s) {
+        constexpr int U = decltype(u_tag)::value;
+        constexpr int CUR = U & 1, PREV = CUR ^ 1;
         {   // request plane s + PF of g
             int zn = s + PF;
             while (zn < 0) zn += N3;
             while (zn >= N3) zn -= N3;
             const VP v = *reinterpret_cast<const VP *>(src + (long)zn * plane + rowoff);
 #pragma unroll
-            for (int px = 0; px < PX; ++px) q[qoff(0, NLEV) + qdepth(0, NLEV) - 1][px] = v[px];
+            for (int px = 0; px < PX; ++px) q[rslot(0, NLEV, U + PF)][px] = v[px];
         }
 #pragma unroll
         for (int l = 0; l < NLEV; ++l) {
             // level l+1 at plane s-l-1 on rows [l+1, rows-l-1): level l at planes
-            // s-l-2 .. s-l (slots qnew-2 .. qnew); the neighbours across the
-            // chunk edges and rows of plane s-l-1 from the previous buffer
-            const int qc = qnew(l, NLEV) - 1;
-            const int qn = qnew(l + 1, NLEV);
+            // s-l-2 .. s-l; the neighbours across the chunk edges and rows of
+            // plane s-l-1 from the previous buffer
+            const int qc = rslot(l, NLEV, U - l - 1);
+            const int qzm = rslot(l, NLEV, U - l - 2), qzp = rslot(l, NLEV, U - l);
+            const int qn = rslot(l + 1, NLEV, U - l - 1);
             if (r >= l + 1 && r < rows - l - 1) {
                 const T *c = pc[PREV][l];
                 const T *cm, *cp;
@@ -212,7 +218,7 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
                     const T right = px == PX - 1 ? xr : q[qc][px + 1];
                     T a = tc0 * cen;
                     a = a + tc1 * (right + left) + tc2 * (yp[px] + ym[px]) +
-                        tc3 * (q[qc + 1][px] + q[qc - 1][px]);
+                        tc3 * (q[qzp][px] + q[qzm][px]);
                     T v;
                     if (l == 0) {
                         v = cen - a;                          // v_0 = g - A g
@@ -220,7 +226,7 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
                         v = ts1 * a - cen;                    // v_1 = tau(A) v_0
                     } else {
                         // v_l = 2 tau(A) v_{l-1} - v_{l-2}; v_{l-2} is level l-1 at this plane
-                        const T prev = q[qnew(l - 1, NLEV) - 2][px];
+                        const T prev = q[rslot(l - 1, NLEV, U - l - 1)][px];
                         v = T(2) * (ts1 * a - cen) - prev;
                     }
                     q[qn][px] = v;
@@ -235,7 +241,7 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
                 for (int px = 0; px < PX; ++px) {
                     T vv[NLEV + 1];
 #pragma unroll
-                    for (int l = 0; l <= NLEV; ++l) vv[l] = q[qout(l, NLEV)][px];
+                    for (int l = 0; l <= NLEV; ++l) vv[l] = q[rslot(l, NLEV, U - NLEV)][px];
                     int e = 0;
 #pragma unroll
                     for (int j = 1; j <= NLEV; ++j)
@@ -253,9 +259,10 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
                 VP outv;
 #pragma unroll
                 for (int px = 0; px < PX; ++px) {
-                    double x = (double)q[qout(0, NLEV)][px];
+                    double x = (double)q[rslot(0, NLEV, U - NLEV)][px];
 #pragma unroll
-                    for (int k = 0; k < NLEV; ++k) x += dk[k] * (double)q[qout(k + 1, NLEV)][px];
+                    for (int k = 0; k < NLEV; ++k)
+                        x += dk[k] * (double)q[rslot(k + 1, NLEV, U - NLEV)][px];
                     // MODE 2: P = Pg + beta P in the working precision, as k_cg_newp
                     T pv = (T)x;
                     if (MODE == 2 && beta) pv = pv + (T)bk * old[px];
@@ -271,24 +278,24 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
                 T *sc = pc[CUR][l] + 1;
                 VP v;
 #pragma unroll
-                for (int px = 0; px < PX; ++px) v[px] = q[qnew(l, NLEV)][px];
+                for (int px = 0; px < PX; ++px) v[px] = q[rslot(l, NLEV, U - l)][px];
                 *reinterpret_cast<VP *>(sc) = v;
                 if (x0 == 0) sc[N1] = v[0];                       // x = N1
                 if (x0 + PX == N1) sc[-x0 - 1] = v[PX - 1];       // x = -1
             }
         }
         __syncthreads();
-        // shift every queue by one plane
-#pragma unroll
-        for (int l = 0; l <= NLEV; ++l)
-#pragma unroll
-            for (int k = 0; k < qdepth(l, NLEV) - 1; ++k)
-#pragma unroll
-                for (int px = 0; px < PX; ++px) q[qoff(l, NLEV) + k][px] = q[qoff(l, NLEV) + k + 1][px];
     };
-    for (int s = s0; s < s1e; s += 2) {
+    static_assert(kRing == 8, "the unrolled steps below");
+    for (int s = s0; s < s1e; s += kRing) {
         step(std::integral_constant<int, 0>{}, s);
         if (s + 1 < s1e) step(std::integral_constant<int, 1>{}, s + 1);
+        if (s + 2 < s1e) step(std::integral_constant<int, 2>{}, s + 2);
+        if (s + 3 < s1e) step(std::integral_constant<int, 3>{}, s + 3);
+        if (s + 4 < s1e) step(std::integral_constant<int, 4>{}, s + 4);
+        if (s + 5 < s1e) step(std::integral_constant<int, 5>{}, s + 5);
+        if (s + 6 < s1e) step(std::integral_constant<int, 6>{}, s + 6);
+        if (s + 7 < s1e) step(std::integral_constant<int, 7>{}, s + 7);
     }
     if (MODE == 0) {
         const int lane = tid & 63, wave = tid >> 6, nw = (blockDim.x + 63) >> 6;
