@@ -1,6 +1,7 @@
 // Symmetric eigensolver of the subspace matrix: Householder reduction to
 // tridiagonal form in one cooperative launch, rocsolver_dstedc on the
-// tridiagonal, and the back-transformation by a wave-per-columns kernel.
+// tridiagonal, and the back-transformation by a blocked wave-per-columns
+// kernel.
 //
 // The reduction is dsytd2 with the lower convention: A = Q T Q^T,
 // Q = H(0) H(1) ... H(n-2), H(i) = I - tau_i v_i v_i^T with v_i(0:i) = 0,
@@ -221,60 +222,147 @@ k_sytrd_coop(const double *__restrict__ A, int n, int cpw, double *__restrict__ 
     if ((n - 1) % nwg == wg && tid == 0) D[n - 1] = cols[(size_t)((n - 1) / nwg) * n + n - 1];
 }
 
-// Z <- Q Z = H(0) H(1) ... H(n-2) Z (dormtr('L', 'L', 'N')): the columns of Z
-// are independent, so each wave carries kQCols of them in registers (NPL
-// values per lane, n <= 64 NPL) through all reflectors, the next reflector
-// loaded while the current one is applied.
-constexpr int kQCols = 2;
-template <int NPL>
-__global__ void __launch_bounds__(256)
-k_apply_q(const double *__restrict__ R, const double *__restrict__ TAU, int n, double *Z)
+// Z <- Q Z = H(0) H(1) ... H(n-2) Z (dormtr('L', 'L', 'N')).  The columns
+// of Z are independent: each wave carries one of them in registers (NPL
+// values per lane, n <= 64 NPL).  The reflectors are applied kQBlk at a time
+// (i = ib .. ib+kQBlk-1, highest first): with d_i = v_i^T z for the column
+// before the block and c_ik = v_i^T v_k,
+//     s_i = tau_i (d_i - sum_{k > i} s_k c_ik),   z <- z - sum_i s_i v_i
+// is the same as applying them one by one, but the kQBlk dot products share
+// one pass over the reflectors and one wave reduction.
+constexpr int kQBlk = 8;
+
+__device__ __forceinline__ double trd_v(const double *R, int n, int i, int r)
 {
-    const int wave = (int)((blockIdx.x * 256u + threadIdx.x) >> 6), lane = threadIdx.x & 63;
-    const int c0 = wave * kQCols;
-    if (c0 >= n) return;
-    double z[kQCols][NPL];
+    return r == i + 1 ? 1.0 : (r > i + 1 && r < n ? R[(size_t)i * n + r] : 0.0);
+}
+
+// C[b][j][k] = v_{ib+j}^T v_{ib+k} for j < k within block b (one wave each)
+__global__ void __launch_bounds__(64)
+k_q_overlaps(const double *__restrict__ R, int n, double *__restrict__ C)
+{
+    const int b = blockIdx.x / (kQBlk * kQBlk), jk = blockIdx.x % (kQBlk * kQBlk);
+    const int j = jk / kQBlk, k = jk % kQBlk, lane = threadIdx.x;
+    const int i = b * kQBlk + j, i2 = b * kQBlk + k;
+    double s = 0.0;
+    if (j < k && i2 <= n - 2)
+        for (int r = lane; r < n; r += 64) s += trd_v(R, n, i, r) * trd_v(R, n, i2, r);
+    s = trd_wave_sum(s);
+    if (lane == 0) C[blockIdx.x] = s;
+}
+
+// Sum of v[0..K) over the wave, K a power of two <= 64: each exchange at
+// distance D keeps half of the values (the half the lane's bit D selects)
+// and adds the partner's copy of it; after log2(K) exchanges lane l holds
+// value l / (64 / K) summed over its group, the remaining distances finish
+// the sum.  Fixed order.
+template <int D, int N, int K>
+__device__ __forceinline__ void trd_reduce_scatter(double (&v)[K], int lane)
+{
+    if constexpr (N > 1) {
+        constexpr int H = N / 2;
+        const bool up = (lane & D) != 0;
 #pragma unroll
-    for (int q = 0; q < kQCols; ++q)
-#pragma unroll
-        for (int m = 0; m < NPL; ++m) {
-            const int r = lane + 64 * m;
-            z[q][m] = (c0 + q < n && r < n) ? Z[(size_t)(c0 + q) * n + r] : 0.0;
+        for (int i = 0; i < H; ++i) {
+            const double send = up ? v[i] : v[i + H];
+            const double keep = up ? v[i + H] : v[i];
+            v[i] = keep + __shfl_xor(send, D, 64);
         }
-    auto load_v = [&](int i, double (&v)[NPL]) {
+        trd_reduce_scatter<D / 2, H, K>(v, lane);
+    }
+}
+
+// One column per wave, kQWaves columns per workgroup; the workgroup stages
+// each block of kQBlk reflectors in LDS (rows padded to 64 NPL), loading the
+// next block from global memory while the current one is applied.
+constexpr int kQWaves = 4;
+template <int NPL>
+__global__ void __launch_bounds__(64 * kQWaves)
+k_apply_q(const double *__restrict__ R, const double *__restrict__ TAU,
+          const double *__restrict__ C, int n, double *Z)
+{
+    constexpr int NP = 64 * NPL;                         // padded reflector length
+    constexpr int NT = 64 * kQWaves;
+    constexpr int SPT = kQBlk * NP / NT;                 // staged values per thread
+    static_assert(SPT * NT == kQBlk * NP, "whole blocks");
+    static_assert(kQBlk <= 64 && (kQBlk & (kQBlk - 1)) == 0, "power of two");
+    extern __shared__ __attribute__((aligned(16))) double Vs[];   // [2][kQBlk][NP]
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int col = blockIdx.x * kQWaves + wave;
+    const bool live = col < n;
+    double z[NPL];
 #pragma unroll
-        for (int m = 0; m < NPL; ++m) {
-            const int r = lane + 64 * m;
-            v[m] = r == i + 1 ? 1.0 : (r > i + 1 && r < n ? R[(size_t)i * n + r] : 0.0);
+    for (int m = 0; m < NPL; ++m) {
+        const int r = lane + 64 * m;
+        z[m] = (live && r < n) ? Z[(size_t)col * n + r] : 0.0;
+    }
+    const int nblk = (n - 1 + kQBlk - 1) / kQBlk;
+    double regs[SPT];
+    auto fetch = [&](int b) {
+        const int ib = b * kQBlk, nk = min(kQBlk, n - 1 - ib);
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            const int e = tid + NT * j, k = e / NP, r = e % NP;
+            regs[j] = k < nk ? trd_v(R, n, ib + k, r) : 0.0;
         }
     };
-    double vc[NPL], vn[NPL];
-    load_v(n - 2, vc);
-    for (int i = n - 2; i >= 0; --i) {
-        if (i > 0) load_v(i - 1, vn);
-        const double t = TAU[i];
-        double s[kQCols];
+    auto put = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < kQCols; ++q) {
-            double a = 0.0;
+        for (int j = 0; j < SPT; ++j) Vs[(size_t)buf * kQBlk * NP + tid + NT * j] = regs[j];
+    };
+    fetch(nblk - 1);
+    put(0);
+    __syncthreads();
+    for (int bi = 0; bi < nblk; ++bi) {
+        const int b = nblk - 1 - bi, cur = bi & 1;
+        if (b > 0) fetch(b - 1);
+        const double *V = Vs + (size_t)cur * kQBlk * NP;
+        if (live) {
+            const int ib = b * kQBlk, nk = min(kQBlk, n - 1 - ib);
+            double a[kQBlk];
 #pragma unroll
-            for (int m = 0; m < NPL; ++m) a += vc[m] * z[q][m];
-            s[q] = t * trd_wave_sum(a);
+            for (int k = 0; k < kQBlk; ++k) {
+                double t = 0.0;
+#pragma unroll
+                for (int m = 0; m < NPL; ++m) t += V[k * NP + lane + 64 * m] * z[m];
+                a[k] = t;
+            }
+            trd_reduce_scatter<32, kQBlk, kQBlk>(a, lane);
+            double rsum = a[0];
+#pragma unroll
+            for (int d = 32 / kQBlk; d > 0; d >>= 1) rsum += __shfl_xor(rsum, d, 64);
+            double sk[kQBlk];
+#pragma unroll
+            for (int k = kQBlk - 1; k >= 0; --k) {
+                const double d = __shfl(rsum, k * (64 / kQBlk), 64);
+                double t = 0.0;
+                if (k < nk) {
+                    t = d;
+#pragma unroll
+                    for (int k2 = k + 1; k2 < kQBlk; ++k2)
+                        if (k2 < nk) t -= sk[k2] * C[((size_t)b * kQBlk + k) * kQBlk + k2];
+                    t *= TAU[ib + k];
+                }
+                sk[k] = t;
+            }
+#pragma unroll
+            for (int m = 0; m < NPL; ++m) {
+                double t = z[m];
+#pragma unroll
+                for (int k = 0; k < kQBlk; ++k) t -= sk[k] * V[k * NP + lane + 64 * m];
+                z[m] = t;
+            }
         }
-#pragma unroll
-        for (int q = 0; q < kQCols; ++q)
-#pragma unroll
-            for (int m = 0; m < NPL; ++m) z[q][m] -= s[q] * vc[m];
-#pragma unroll
-        for (int m = 0; m < NPL; ++m) vc[m] = vn[m];
+        if (b > 0) put(cur ^ 1);
+        __syncthreads();
     }
-#pragma unroll
-    for (int q = 0; q < kQCols; ++q)
+    if (live) {
 #pragma unroll
         for (int m = 0; m < NPL; ++m) {
             const int r = lane + 64 * m;
-            if (c0 + q < n && r < n) Z[(size_t)(c0 + q) * n + r] = z[q][m];
+            if (r < n) Z[(size_t)col * n + r] = z[m];
         }
+    }
 }
 
 __global__ void k_trd_clear(unsigned *flags, int nflags, int *err)
@@ -326,9 +414,13 @@ int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *
     if (!trd_shape(n, &nwg, &cpw, &lds)) return kOK;
     const size_t nn = (size_t)n * n;
     const int nflags = 16 * nwg;
-    RS_TRY(c.evwork.ensure(sizeof(double) * (nn + 6 * (size_t)n) + sizeof(int) * (nflags + 4)));
+    const int nqb = (n - 1 + kQBlk - 1) / kQBlk;
+    const size_t nc = (size_t)nqb * kQBlk * kQBlk;
+    RS_TRY(c.evwork.ensure(sizeof(double) * (nn + 6 * (size_t)n + nc) +
+                           sizeof(int) * (nflags + 4)));
     double *R = c.evwork.as<double>(), *E = R + nn, *TAU = E + n, *pub = TAU + n;
-    unsigned *flags = reinterpret_cast<unsigned *>(pub + 4 * (size_t)n);
+    double *Cq = pub + 4 * (size_t)n;
+    unsigned *flags = reinterpret_cast<unsigned *>(Cq + nc);
     int *err = reinterpret_cast<int *>(flags + nflags);
     if (lds > 64 * 1024)
         RS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_sytrd_coop),
@@ -339,12 +431,25 @@ int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *
     RS_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_sytrd_coop), dim3(nwg),
                                       dim3(kTrdThreads), args, (unsigned)lds, c.stream));
     RS_BLAS(rocsolver_dstedc(c.blas, rocblas_evect_tridiagonal, n, D, E, V, n, einfo));
-    const int waves = (n + kQCols - 1) / kQCols;
-    const dim3 qgrid((waves + 3) / 4), qblk(256);
-    if (n <= 64) hipLaunchKernelGGL((k_apply_q<1>), qgrid, qblk, 0, c.stream, R, TAU, n, V);
-    else if (n <= 128) hipLaunchKernelGGL((k_apply_q<2>), qgrid, qblk, 0, c.stream, R, TAU, n, V);
-    else if (n <= 256) hipLaunchKernelGGL((k_apply_q<4>), qgrid, qblk, 0, c.stream, R, TAU, n, V);
-    else if (n <= 512) hipLaunchKernelGGL((k_apply_q<8>), qgrid, qblk, 0, c.stream, R, TAU, n, V);
+    const dim3 qgrid((n + kQWaves - 1) / kQWaves), qblk(64 * kQWaves);
+    const size_t qlds = sizeof(double) * 2 * kQBlk * 64;   // times NPL
+    if (n <= 512) {
+        if (8 * qlds > 64 * 1024)
+            RS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_apply_q<8>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)(8 * qlds)));
+        if (4 * qlds > 64 * 1024)
+            RS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_apply_q<4>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)(4 * qlds)));
+        hipLaunchKernelGGL(k_q_overlaps, dim3((unsigned)nc), dim3(64), 0, c.stream, R, n, Cq);
+    }
+    if (n <= 64)
+        hipLaunchKernelGGL((k_apply_q<1>), qgrid, qblk, qlds, c.stream, R, TAU, Cq, n, V);
+    else if (n <= 128)
+        hipLaunchKernelGGL((k_apply_q<2>), qgrid, qblk, 2 * qlds, c.stream, R, TAU, Cq, n, V);
+    else if (n <= 256)
+        hipLaunchKernelGGL((k_apply_q<4>), qgrid, qblk, 4 * qlds, c.stream, R, TAU, Cq, n, V);
+    else if (n <= 512)
+        hipLaunchKernelGGL((k_apply_q<8>), qgrid, qblk, 8 * qlds, c.stream, R, TAU, Cq, n, V);
     else
         RS_BLAS(rocsolver_dormtr(c.blas, rocblas_side_left, rocblas_fill_lower,
                                  rocblas_operation_none, n, n, R, n, TAU, V, n));
