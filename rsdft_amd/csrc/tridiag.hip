@@ -30,6 +30,7 @@
 // the workgroup signals arrival, so a workgroup that sees all arrival flags
 // reads complete data.
 #include "common.h"
+#include "coop.h"
 
 #include <algorithm>
 
@@ -38,19 +39,11 @@ namespace rsdft {
 constexpr int kTrdThreads = 256;
 constexpr int kTrdWaves = kTrdThreads / 64;
 constexpr int kTrdMaxWG = 128;
-constexpr unsigned kTrdSpinLimit = 1u << 24;   // polls before a barrier gives up (seconds)
-
-__device__ __forceinline__ double trd_wave_sum(double x)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;   // the same value in every lane (addition commutes)
-}
 
 // sum over the workgroup, the same value in every thread; red: kTrdWaves doubles
 __device__ __forceinline__ double trd_block_sum(double x, double *red)
 {
-    x = trd_wave_sum(x);
+    x = coop_wave_sum(x);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
     __syncthreads();
@@ -82,54 +75,6 @@ __device__ void trd_householder(const double *col, double *v, int n, int s, doub
     for (int i = threadIdx.x; i < n; i += kTrdThreads)
         v[i] = i <= s ? 0.0 : (i == s + 1 ? 1.0 : col[i] * scal);
     __syncthreads();
-}
-
-// Grid barrier on per-workgroup arrival flags (16 words apart): workgroup g
-// stores the step count in flags[16 g] once its stores have completed; the
-// first wave polls every flag in parallel.  A wait beyond the poll limit, or
-// an error raised by another workgroup, sets *err and makes every workgroup
-// leave; returns false then.
-__device__ bool trd_grid_barrier(unsigned *flags, unsigned target, int *err, int *stop)
-{
-    __builtin_amdgcn_s_waitcnt(0);   // this wave's publication stores have completed
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x, nwg = gridDim.x;
-        if (lane == 0)
-            __hip_atomic_store(flags + 16 * blockIdx.x, target, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        unsigned polls = 0;
-        while (true) {
-            bool ok = true;
-            for (int g = lane; g < nwg; g += 64)
-                ok = ok && __hip_atomic_load(flags + 16 * g, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT) >= target;
-            if (__all(ok)) break;
-            const bool bad =
-                __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-                ++polls > kTrdSpinLimit;
-            if (__any(bad)) {
-                if (lane == 0) {
-                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    *stop = 1;
-                }
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __syncthreads();
-    return *stop == 0;
-}
-
-__device__ __forceinline__ void trd_put(double *p, double x)
-{
-    __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ double trd_get(const double *p)
-{
-    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // A (n x n, symmetric, full storage) -> D, E, TAU, reflectors in R.
@@ -177,20 +122,20 @@ k_sytrd_coop(const double *__restrict__ A, int n, int cpw, double *__restrict__ 
             const double *a = cols + (size_t)c * n;
             double s = 0.0;
             for (int i = k + 1 + lane; i < n; i += 64) s += a[i] * v[i];
-            s = tau * trd_wave_sum(s);
-            if (lane == 0) trd_put(pp + j, s);
+            s = tau * coop_wave_sum(s);
+            if (lane == 0) coop_put(pp + j, s);
         }
         if ((k + 1) % nwg == wg) {
             const double *a = cols + (size_t)((k + 1) / nwg) * n;
-            for (int i = k + 1 + tid; i < n; i += kTrdThreads) trd_put(pc + i, a[i]);
+            for (int i = k + 1 + tid; i < n; i += kTrdThreads) coop_put(pc + i, a[i]);
         }
-        if (!trd_grid_barrier(flags, (unsigned)(k + 1), err, &stop)) return;
+        if (!coop_grid_barrier(flags, (unsigned)(k + 1), err, &stop)) return;
         // b. the whole p and the column k+1; p^T v
         double t = 0.0;
         for (int i = k + 1 + tid; i < n; i += kTrdThreads) {
-            const double pi = trd_get(pp + i);
+            const double pi = coop_get(pp + i);
             w[i] = pi;
-            col[i] = trd_get(pc + i);
+            col[i] = coop_get(pc + i);
             t += pi * v[i];
         }
         const double alpha = -0.5 * tau * trd_block_sum(t, red);   // ends with a barrier
@@ -247,7 +192,7 @@ k_q_overlaps(const double *__restrict__ R, int n, double *__restrict__ C)
     double s = 0.0;
     if (j < k && i2 <= n - 2)
         for (int r = lane; r < n; r += 64) s += trd_v(R, n, i, r) * trd_v(R, n, i2, r);
-    s = trd_wave_sum(s);
+    s = coop_wave_sum(s);
     if (lane == 0) C[blockIdx.x] = s;
 }
 
