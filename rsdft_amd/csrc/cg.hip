@@ -18,9 +18,12 @@ int pc_stencil(const T *in, T *Q, T *R, T *PP, T *PG, int nb, int mode, const in
                double *part);
 
 constexpr int kChunkThreads = 256;
-constexpr int kChunkElems = 2048;
+// elements of one band per chunk: 32 KB of each array (large enough that the
+// number of workgroups does not limit the streaming rate)
+template <class T> constexpr int kChunk = 32768 / (int)sizeof(T);
 
-static inline int nchunk_of(long ML) { return (int)((ML + kChunkElems - 1) / kChunkElems); }
+template <class T>
+static inline int nchunk_of(long ML) { return (int)((ML + kChunk<T> - 1) / kChunk<T>); }
 
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -51,23 +54,69 @@ struct PtrSet {
 };
 
 // part[(k*nb + b)*nchunk + chunk] = sum over the chunk of a_k*b_k (band b)
+// 16-byte vectors of T; ML is a multiple of 4 (N1 is), so every band and
+// chunk starts 16-byte aligned and holds whole vectors
+template <class T> struct Vec16 { typedef T type __attribute__((ext_vector_type(16 / sizeof(T)))); };
+template <class T> constexpr int kVW = 16 / (int)sizeof(T);
+
 template <class T, int ND>
 __global__ void __launch_bounds__(kChunkThreads)
 k_dots(PtrSet<T, ND> p, long ML, int nb, int nchunk, double *part, const int *active)
 {
+    typedef typename Vec16<T>::type V;
+    constexpr int VW = kVW<T>;
     const int b = blockIdx.y, chunk = blockIdx.x;
     if (active && !active[b]) return;
     double s[ND];
 #pragma unroll
     for (int k = 0; k < ND; ++k) s[k] = 0.0;
     const long off = (long)b * ML;
-    const long i0 = (long)chunk * kChunkElems;
-    const long i1 = min(ML, i0 + kChunkElems);
-    for (long i = i0 + threadIdx.x; i < i1; i += kChunkThreads) {
+    const long i0 = (long)chunk * kChunk<T>;
+    const long i1 = min(ML, i0 + kChunk<T>);
+    for (long i = i0 + (long)threadIdx.x * VW; i < i1; i += (long)kChunkThreads * VW) {
 #pragma unroll
-        for (int k = 0; k < ND; ++k) s[k] += (double)p.a[k][off + i] * (double)p.b[k][off + i];
+        for (int k = 0; k < ND; ++k) {
+            const V x = *reinterpret_cast<const V *>(p.a[k] + off + i);
+            const V y = *reinterpret_cast<const V *>(p.b[k] + off + i);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) s[k] += (double)x[e] * (double)y[e];
+        }
     }
     block_sum_write<ND>(s, part + ((long)b) * nchunk + chunk, (long)nb * nchunk);
+}
+
+// The six inner products of the CG step (cg_module.f90:283-296) from four
+// arrays, each read once: (x,x), (p,x), (p,p), (x,Hx), (p,Hx), (p,Hp)
+template <class T>
+__global__ void __launch_bounds__(kChunkThreads)
+k_dots_cg(const T *__restrict__ X, const T *__restrict__ P, const T *__restrict__ HX,
+          const T *__restrict__ HP, long ML, int nb, int nchunk, double *part, const int *active)
+{
+    typedef typename Vec16<T>::type V;
+    constexpr int VW = kVW<T>;
+    const int b = blockIdx.y, chunk = blockIdx.x;
+    if (active && !active[b]) return;
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const long off = (long)b * ML;
+    const long i0 = (long)chunk * kChunk<T>;
+    const long i1 = min(ML, i0 + kChunk<T>);
+    for (long i = i0 + (long)threadIdx.x * VW; i < i1; i += (long)kChunkThreads * VW) {
+        const V x = *reinterpret_cast<const V *>(X + off + i);
+        const V pv = *reinterpret_cast<const V *>(P + off + i);
+        const V hx = *reinterpret_cast<const V *>(HX + off + i);
+        const V hp = *reinterpret_cast<const V *>(HP + off + i);
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            const double xe = x[e], pe = pv[e], hxe = hx[e], hpe = hp[e];
+            s[0] += xe * xe;
+            s[1] += pe * xe;
+            s[2] += pe * pe;
+            s[3] += xe * hxe;
+            s[4] += pe * hxe;
+            s[5] += pe * hpe;
+        }
+    }
+    block_sum_write<6>(s, part + ((long)b) * nchunk + chunk, (long)nb * nchunk);
 }
 
 // sum of partials in fixed order
@@ -290,8 +339,8 @@ __global__ void k_pc_beta(CGView v, const double *part, int nb, int nchunk)
 #define BAND_LOOP_BEGIN                                                       \
     const int b = blockIdx.y, chunk = blockIdx.x;                             \
     const long off = (long)b * ML;                                            \
-    const long i0 = (long)chunk * kChunkElems;                                \
-    const long i1 = min(ML, i0 + kChunkElems);
+    const long i0 = (long)chunk * kChunk<T>;                                \
+    const long i1 = min(ML, i0 + kChunk<T>);
 
 // g = -2 (Hx - E x); part = sum g^2
 template <class T>
@@ -300,12 +349,20 @@ k_cg_grad(const T *HX, const T *X, T *G, CGView v, long ML, int nchunk, double *
 {
     BAND_LOOP_BEGIN
     if (!v.active[b]) return;
+    typedef typename Vec16<T>::type V;
+    constexpr int VW = kVW<T>;
     const T E = (T)v.E[b];
     double s[1] = {0.0};
-    for (long i = i0 + threadIdx.x; i < i1; i += kChunkThreads) {
-        const T g = T(-2) * (HX[off + i] - E * X[off + i]);
-        G[off + i] = g;
-        s[0] += (double)g * (double)g;
+    for (long i = i0 + (long)threadIdx.x * VW; i < i1; i += (long)kChunkThreads * VW) {
+        const V hx = *reinterpret_cast<const V *>(HX + off + i);
+        const V x = *reinterpret_cast<const V *>(X + off + i);
+        V g;
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            g[e] = T(-2) * (hx[e] - E * x[e]);
+            s[0] += (double)g[e] * (double)g[e];
+        }
+        *reinterpret_cast<V *>(G + off + i) = g;
     }
     block_sum_write<1>(s, part + (long)b * nchunk + chunk, 0);
 }
@@ -340,17 +397,27 @@ k_cg_update(T *HX, const T *HP, T *X, const T *P, T *G, CGView v, long ML, int n
 {
     BAND_LOOP_BEGIN
     if (!v.active[b]) return;
+    typedef typename Vec16<T>::type V;
+    constexpr int VW = kVW<T>;
     const T u1 = (T)v.u1[b], u2 = (T)v.u2[b], w = (T)v.W[b];
     const bool spec = cg_speculative(v.u1[b]);
     double s[1] = {0.0};
-    for (long i = i0 + threadIdx.x; i < i1; i += kChunkThreads) {
-        const T hx = u1 * HX[off + i] + u2 * HP[off + i];
-        HX[off + i] = hx;
-        const T xn = u1 * X[off + i] + u2 * P[off + i];
-        const T g = T(-2) * (hx - w * xn);
-        G[off + i] = g;
-        if (spec) X[off + i] = xn;
-        s[0] += (double)g * (double)g;
+    for (long i = i0 + (long)threadIdx.x * VW; i < i1; i += (long)kChunkThreads * VW) {
+        const V hx0 = *reinterpret_cast<const V *>(HX + off + i);
+        const V hp = *reinterpret_cast<const V *>(HP + off + i);
+        const V x0 = *reinterpret_cast<const V *>(X + off + i);
+        const V pv = *reinterpret_cast<const V *>(P + off + i);
+        V hx, xn, g;
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            hx[e] = u1 * hx0[e] + u2 * hp[e];
+            xn[e] = u1 * x0[e] + u2 * pv[e];
+            g[e] = T(-2) * (hx[e] - w * xn[e]);
+            s[0] += (double)g[e] * (double)g[e];
+        }
+        *reinterpret_cast<V *>(HX + off + i) = hx;
+        *reinterpret_cast<V *>(G + off + i) = g;
+        if (spec) *reinterpret_cast<V *>(X + off + i) = xn;
     }
     block_sum_write<1>(s, part + (long)b * nchunk + chunk, 0);
 }
@@ -445,7 +512,7 @@ int band_dots(int ndot, const T *const *a, const T *const *b, int nb, double sca
 {
     Context &c = ctx();
     const long ML = c.g.ML;
-    const int nch = nchunk_of(ML);
+    const int nch = nchunk_of<T>(ML);
     RS_TRY(c.part.ensure(sizeof(double) * (size_t)6 * nb * nch));
     double *part = c.part.as<double>();
     dim3 grid(nch, nb);
@@ -479,7 +546,7 @@ int precond_batched(const T *g, T *pg, int nb, const int *active)
 {
     Context &c = ctx();
     const long ML = c.g.ML;
-    const int nch = nchunk_of(ML);
+    const int nch = nchunk_of<T>(ML);
     CGView v = view(c.cg);
     const dim3 bgrid((nb + 127) / 128), bblk(128);
     const dim3 egrid(nch, nb), eblk(kChunkThreads);
@@ -520,7 +587,7 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
     Context &c = ctx();
     const long ML = c.g.ML;
     const double dV = c.g.dV;
-    const int nch = nchunk_of(ML);
+    const int nch = nchunk_of<T>(ML);
     RS_TRY(c.cg.ensure(nb));
     // one size for every reduction of the sweep, so no later ensure() moves it
     const int maxparts = std::max(nch, c.geom_pc.tiles);
@@ -571,8 +638,8 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
         }
         RS_TRY(hpsi_apply<T>(P, HP, nb, kKin | kLoc | kNl, act));
         {
-            const T *a[6] = {X, P, P, X, P, P}, *bb[6] = {X, X, P, HX, HX, HP};
-            RS_TRY(band_dots<T>(6, a, bb, nb, 1.0, nullptr, act));
+            hipLaunchKernelGGL((k_dots_cg<T>), egrid, eblk, 0, c.stream, X, P, HX, HP, ML, nb, nch,
+                               part, act);
             collapse_parts(part, 6L * nb, nch);
             hipLaunchKernelGGL(k_cg_2x2, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV);
         }
