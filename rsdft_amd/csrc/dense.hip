@@ -19,7 +19,8 @@ namespace rsdft {
 
 int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *done);
 template <class T>
-int split_gemm_tn(const float *A, const float *B, int nb, long K, double alpha, double *C);
+int split_gemm_tn(const float *A, const float *B, int nb, long K, double alpha, double *C,
+                  int sym_result);
 template <class T>
 int split_gemm_nn(const float *A, const double *U, int nb, long M, float *C, int fp32);
 
@@ -499,9 +500,12 @@ static int refine_eigh(const double *H, int n, double *V, double *D, int *ok)
 }
 
 // C(nb x nb) = alpha * A^T B, A and B (ML x nb) column-major, result FP64
-// (rocblas_dsyrk for A == B measured ~10x slower than dgemm at 480 x 373,248)
+// (rocblas_dsyrk for A == B measured ~10x slower than dgemm at 480 x 373,248).
+// sym_result: A^T B is symmetric in exact arithmetic; the FP32 kernels then
+// compute the tile pairs i <= j only and mirror them
 template <class T>
-static int gemm_tn(const T *A, const T *B, int nb, double alpha, double *C)
+static int gemm_tn(const T *A, const T *B, int nb, double alpha, double *C,
+                   bool sym_result = false)
 {
     Context &c = ctx();
     const long ML = c.g.ML;
@@ -512,7 +516,8 @@ static int gemm_tn(const T *A, const T *B, int nb, double alpha, double *C)
                               &z, C, nb));
         return kOK;
     }
-    return split_gemm_tn<T>((const float *)A, (const float *)B, nb, ML, alpha, C);
+    return split_gemm_tn<T>((const float *)A, (const float *)B, nb, ML, alpha, C,
+                            sym_result ? 1 : 0);
 }
 
 // Out(ML x nb) = A(ML x nb) * U(nb x nb), U FP64; fp32: FP32 products in the
@@ -648,7 +653,9 @@ int rayleigh_ritz_factored_dev(T *X, const T *HX, int nb, double *esp_dev)
     double *Ri = c.mat.as<double>(), *C = c.mat2.as<double>(), *H = c.mat3.as<double>(),
            *V4 = c.mat4.as<double>();
     const double one = 1.0, zero = 0.0;
-    RS_TRY(gemm_tn<T>(X, HX, nb, 1.0, C));
+    // X^T HX is symmetric (H is) and only 0.5 (C + C^T) is used: the upper
+    // tile pairs suffice, the difference is the rounding-level antisymmetric part
+    RS_TRY(gemm_tn<T>(X, HX, nb, 1.0, C, true));
     hipLaunchKernelGGL(k_symmetrize, dim3((nb + 127) / 128, nb), dim3(128), 0, c.stream, C, H, nb,
                        0.5 * c.g.dV);
     RS_HIP(hipGetLastError());

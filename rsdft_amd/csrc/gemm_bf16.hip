@@ -373,13 +373,16 @@ __global__ void k_to_f32(const double *src, float *dst, long n)
 // ---------------------------------------------------------------------------
 // host wrappers
 // ---------------------------------------------------------------------------
+// sym_result: A^T B is known to be symmetric (A == B, or B = H A with H
+// symmetric): only the tile pairs i <= j are computed, the rest mirrored
 template <int MODE>
-static int gemm_tn_mode(const float *A, const float *B, int nb, long K, double alpha, double *C)
+static int gemm_tn_mode(const float *A, const float *B, int nb, long K, double alpha, double *C,
+                        bool sym_result)
 {
     Context &c = ctx();
     constexpr int BB = 2, TILE = 32 * BB;
     const int nt = (nb + TILE - 1) / TILE;
-    const bool sym = A == B;
+    const bool sym = A == B || sym_result;
     const int npairs = sym ? nt * (nt + 1) / 2 : nt * nt;
     // enough workgroups to fill 256 CUs several times over
     int nchunk = (int)std::max<long>(1, std::min<long>((4096 + npairs - 1) / npairs, (K + 4095) / 4096));
@@ -437,10 +440,11 @@ static int gemm_nn_mode(const float *A, const double *U, int nb, long M, float *
 }
 
 template <class T>
-int split_gemm_tn(const float *A, const float *B, int nb, long K, double alpha, double *C)
+int split_gemm_tn(const float *A, const float *B, int nb, long K, double alpha, double *C,
+                  int sym_result)
 {
-    return ctx().prec == 2 ? gemm_tn_mode<2>(A, B, nb, K, alpha, C)
-                           : gemm_tn_mode<1>(A, B, nb, K, alpha, C);
+    return ctx().prec == 2 ? gemm_tn_mode<2>(A, B, nb, K, alpha, C, sym_result != 0)
+                           : gemm_tn_mode<1>(A, B, nb, K, alpha, C, sym_result != 0);
 }
 
 // fp32 != 0: FP32 products whatever the precision mode
@@ -451,8 +455,8 @@ int split_gemm_nn(const float *A, const double *U, int nb, long M, float *Out, i
                                     : gemm_nn_mode<1>(A, U, nb, M, Out);
 }
 
-template int split_gemm_tn<float>(const float *, const float *, int, long, double, double *);
-template int split_gemm_tn<double>(const float *, const float *, int, long, double, double *);
+template int split_gemm_tn<float>(const float *, const float *, int, long, double, double *, int);
+template int split_gemm_tn<double>(const float *, const float *, int, long, double, double *, int);
 template int split_gemm_nn<float>(const float *, const double *, int, long, float *, int);
 template int split_gemm_nn<double>(const float *, const double *, int, long, float *, int);
 
