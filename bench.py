@@ -46,23 +46,22 @@ def log(msg):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hpsi_traffic_si216.json")
 
 
-def covered_points(sysm):
-    """Grid points inside some projector sphere."""
-    P = sysm.proj
-    pts = [P.JJP[l, :P.MJJ[l]] for l in range(P.nzlma)]
-    return int(np.unique(np.concatenate(pts)).size) if pts else 0
-
-
-def hpsi_stencil_bytes(sysm, nb, s, fused_nl):
+def hpsi_stencil_bytes(sysm, nb, s, acc):
     """Algorithmic HBM bytes of one all-band launch of the stencil kernel:
-    psi read + H psi write per band and Vloc once; with the fused nonlocal
-    epilogue also the compact nonlocal term of every band and the covered-point
-    index once."""
+    psi read + H psi write per band and Vloc once; accumulating (FP32 H psi)
+    also the read of the nonlocal term that the grid-order fill wrote into
+    H psi, one more band array."""
     ML = sysm.grid.ML
     b = 2 * ML * nb * s + ML * s
-    if fused_nl:
-        b += covered_points(sysm) * nb * s + ML * 4
+    if acc:
+        b += ML * nb * s
     return b
+
+
+def nl_fill_bytes(sysm, nb):
+    """Algorithmic HBM bytes of one all-band launch of the FP32 grid-order
+    nonlocal fill: the term written at every grid point of every band."""
+    return sysm.grid.ML * nb * 4
 
 
 def pmc_summary(fetch_dir, write_dir, out_path):
@@ -87,7 +86,7 @@ def pmc_summary(fetch_dir, write_dir, out_path):
     res = {}
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         vals = load(d, counter)
-        kern = [k for k in vals if re.search(r"k_hpsi2<float, \d+, \d+, \d+, 3,", k)]
+        kern = [k for k in vals if re.search(r"k_hpsi2<float, \d+, \d+, \d+, 11,", k)]
         copy = [k for k in vals if "k_copy_bands<float>" in k]
         if not kern or not copy:
             raise SystemExit(f"{counter}: kernels not found in {d}")
@@ -98,7 +97,7 @@ def pmc_summary(fetch_dir, write_dir, out_path):
                         "copy_bytes_algorithmic": copy_bytes, "scale": scale,
                         "kernel_bytes": kv * scale, "dispatches": len(vals[kern[0]])}
     res["traffic_bytes"] = res["FETCH_SIZE"]["kernel_bytes"] + res["WRITE_SIZE"]["kernel_bytes"]
-    res["algorithmic_bytes"] = hpsi_stencil_bytes(sysm, nb, 4, False)
+    res["algorithmic_bytes"] = hpsi_stencil_bytes(sysm, nb, 4, True)
     res["commands"] = [
         "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc FETCH_SIZE "
         "-d <repo>/gpurun_out/pmc_fetch -o run -f csv -- python <repo>/bench.py --traffic-probe",
@@ -267,17 +266,21 @@ def main():
     log(f"FP64 from the FP32-rounded start: max {flo_err:.3e} Ha (occupied {flo_err_occ:.3e}), "
         f"band energy {flo_eband:.3e}")
 
-    # ---- roofline of the dominant kernel (fused hpsi stencil, FP32) ---------
+    # ---- roofline of the dominant kernel (hpsi stencil, FP32) ---------------
     t_kernel_ms = float(hpm[4])
-    B = hpsi_stencil_bytes(sysm, nb, 4, False)
+    B = hpsi_stencil_bytes(sysm, nb, 4, True)
     achieved = B / (t_kernel_ms * 1e-3) / 1e9
     B64 = hpsi_stencil_bytes(sysm, nb, 8, False)
     ach64 = B64 / (float(hp64[4]) * 1e-3) / 1e9
     copy_gbs = 2 * sysm.grid.ML * nb * 4 / (float(hpm[7]) * 1e-3) / 1e9
+    fill_gbs = nl_fill_bytes(sysm, nb) / (float(hpm[6]) * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = float(json.load(f)["traffic_bytes"])
+            tr = json.load(f)
+        # counters of the kernel timed here (the accumulating FP32 stencil) only
+        if ", 11," in tr["FETCH_SIZE"]["kernel"]:
+            traffic = float(tr["traffic_bytes"])
 
     cpu = None
     if not args.no_cpu:
@@ -312,6 +315,7 @@ def main():
             "hpsi_ms_fp32": float(hpm[0]),
             "hpsi_ms_fp64": float(hp64[0]),
             "hpsi_stencil_gbs_fp64": ach64,
+            "nl_fill_gbs_fp32": fill_gbs,
             "hpsi_parts_ms_fp32": {k: float(v) for k, v in zip(
                 ("hamiltonian", "kinetic", "localpot", "nonlocal", "stencil", "projection",
                  "nl_stage", "copy"), hpm)},
@@ -323,7 +327,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_hpsi2<float,6,2,1,kin|loc,1024> (Md=6 FD stencil + Vloc, all bands)",
+            "kernel": "k_hpsi2<float,6,2,1,kin|loc|acc,1024> (Md=6 FD stencil + Vloc added to "
+                      "the nonlocal term in H psi, all bands)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -378,25 +378,51 @@ int rsdft_gpu_set_projectors(int nzlma, int maxmjj, const int *MJJ, const int *J
             if (gcol[g] == col) order.push_back(g);
         nl.color_ptr.push_back((int)order.size());
     }
-    // covered points, their compact index and group memberships
-    std::vector<int> cidx(ML, -1), mcount;
-    int ncov = 0;
-    for (long k = 0; k < npts; ++k) {
-        int &ci = cidx[gidx[k]];
-        if (ci < 0) ci = -2;   // mark
-    }
-    for (long i = 0; i < ML; ++i)
-        if (cidx[i] == -2) cidx[i] = ncov++;
-    std::vector<int> mptr(ncov + 1, 0);
-    for (long k = 0; k < npts; ++k) mptr[cidx[gidx[k]] + 1]++;
-    for (int q = 0; q < ncov; ++q) mptr[q + 1] += mptr[q];
-    std::vector<int> mpos(npts), mg(npts), mfill(mptr.begin(), mptr.end() - 1);
-    for (int g = 0; g < ng; ++g)   // memberships of a point in group order
+    // grid-order fill tables: groups touching each range of kFillRange points
+    // (in group order) and, per grid point, its memberships as range-local slots
+    const int nrange = (int)((ML + kFillRange - 1) / kFillRange);
+    std::vector<int> rptr(nrange + 1, 0), rlast(nrange, -1), rgrp;
+    std::vector<std::vector<int>> rlist(nrange);
+    for (int g = 0; g < ng; ++g)
         for (int k = gptr[g]; k < gptr[g + 1]; ++k) {
-            const int e = mfill[cidx[gidx[k]]]++;
-            mpos[e] = k;
-            mg[e] = g;
+            const int r = gidx[k] / kFillRange;
+            if (rlast[r] != g) { rlast[r] = g; rlist[r].push_back(g); }
         }
+    int gmax = 0;
+    for (int r = 0; r < nrange; ++r) {
+        rptr[r + 1] = rptr[r] + (int)rlist[r].size();
+        gmax = std::max(gmax, (int)rlist[r].size());
+        rgrp.insert(rgrp.end(), rlist[r].begin(), rlist[r].end());
+    }
+    std::vector<uint32_t> meta(ML, 0u);
+    int mmax = 0;
+    for (int g = 0; g < ng; ++g)
+        for (int k = gptr[g]; k < gptr[g + 1]; ++k) {
+            const int i = gidx[k];
+            const int n = (int)(meta[i] & 7u);
+            mmax = std::max(mmax, n + 1);
+            if (n >= kFillMaxM) continue;   // layout rejected below
+            const auto &L = rlist[i / kFillRange];
+            const int slot = (int)(std::find(L.begin(), L.end(), g) - L.begin());
+            meta[i] = (meta[i] & ~7u) | (uint32_t)(n + 1) | ((uint32_t)slot << (3 + 6 * n));
+        }
+    int fill_mm = 0;
+    if (mmax <= kFillMaxM && gmax <= kFillMaxG && ML % 4 == 0 && npts > 0) {
+        fill_mm = 1;
+        while (fill_mm < mmax) fill_mm *= 2;
+        if (fill_mm * gp > 16) fill_mm = 0;   // per-thread uV registers
+    }
+    std::vector<float> fuv;
+    if (fill_mm) {
+        fuv.assign((size_t)fill_mm * ML * gp, 0.0f);
+        std::vector<uint8_t> seen(ML, 0);
+        for (int g = 0; g < ng; ++g)
+            for (int k = gptr[g]; k < gptr[g + 1]; ++k) {
+                const int i = gidx[k], m = seen[i]++;
+                for (int s2 = 0; s2 < gp; ++s2)
+                    fuv[((size_t)m * ML + i) * gp + s2] = (float)gval[(size_t)k * gp + s2];
+            }
+    }
     std::vector<float> gval32(gval.begin(), gval.end());
     std::vector<double> scale(nzlma);
     for (int l = 0; l < nzlma; ++l) scale[l] = (double)iuV[l] * c.g.dV;
@@ -415,17 +441,20 @@ int rsdft_gpu_set_projectors(int nzlma, int maxmjj, const int *MJJ, const int *J
         RS_HIP(hipMemcpy(nl.g_val32.p, gval32.data(), sizeof(float) * gval.size(), hipMemcpyHostToDevice));
     }
     RS_HIP(hipMemcpy(nl.g_order.p, order.data(), sizeof(int) * ng, hipMemcpyHostToDevice));
-    RS_TRY(nl.cov_index.ensure(sizeof(int) * ML));
-    RS_TRY(nl.mem_ptr.ensure(sizeof(int) * (ncov + 1)));
-    RS_TRY(nl.mem_pos.ensure(sizeof(int) * std::max<long>(npts, 1)));
-    RS_TRY(nl.mem_g.ensure(sizeof(int) * std::max<long>(npts, 1)));
-    RS_HIP(hipMemcpy(nl.cov_index.p, cidx.data(), sizeof(int) * ML, hipMemcpyHostToDevice));
-    RS_HIP(hipMemcpy(nl.mem_ptr.p, mptr.data(), sizeof(int) * (ncov + 1), hipMemcpyHostToDevice));
-    if (npts) {
-        RS_HIP(hipMemcpy(nl.mem_pos.p, mpos.data(), sizeof(int) * npts, hipMemcpyHostToDevice));
-        RS_HIP(hipMemcpy(nl.mem_g.p, mg.data(), sizeof(int) * npts, hipMemcpyHostToDevice));
+    nl.fill_mm = fill_mm;
+    nl.fill_gmax = gmax;
+    nl.nrange = nrange;
+    if (fill_mm) {
+        RS_TRY(nl.fill_rptr.ensure(sizeof(int) * (nrange + 1)));
+        RS_TRY(nl.fill_rgrp.ensure(sizeof(int) * std::max<size_t>(rgrp.size(), 1)));
+        RS_TRY(nl.fill_meta.ensure(sizeof(uint32_t) * ML));
+        RS_TRY(nl.fill_uv.ensure(sizeof(float) * fuv.size()));
+        RS_HIP(hipMemcpy(nl.fill_rptr.p, rptr.data(), sizeof(int) * (nrange + 1), hipMemcpyHostToDevice));
+        if (!rgrp.empty())
+            RS_HIP(hipMemcpy(nl.fill_rgrp.p, rgrp.data(), sizeof(int) * rgrp.size(), hipMemcpyHostToDevice));
+        RS_HIP(hipMemcpy(nl.fill_meta.p, meta.data(), sizeof(uint32_t) * ML, hipMemcpyHostToDevice));
+        RS_HIP(hipMemcpy(nl.fill_uv.p, fuv.data(), sizeof(float) * fuv.size(), hipMemcpyHostToDevice));
     }
-    nl.ncov = ncov;
     RS_HIP(hipMemcpy(nl.scale64.p, scale.data(), sizeof(double) * nzlma, hipMemcpyHostToDevice));
     nl.nz = nzlma;
     nl.ng = ng;
@@ -872,7 +901,7 @@ int rsdft_gpu_finalize(void)
                      &c.mat3, &c.mat4, &c.eval, &c.einfo, &c.evwork, &c.matT, &c.dots, &c.gemm_part, &c.opbuf, &c.pc_coef,
                      &c.cg.buf, &c.cg.flags, &c.nl.g_lma, &c.nl.g_ptr, &c.nl.g_idx, &c.nl.g_val64,
                      &c.nl.g_val32, &c.nl.g_order, &c.nl.scale64, &c.nl.d_ptr, &c.nl.d_idx, &c.nl.d_val,
-                     &c.nl.cov_index, &c.nl.mem_ptr, &c.nl.mem_pos, &c.nl.mem_g, &c.nlbuf,
+                     &c.nl.fill_rptr, &c.nl.fill_rgrp, &c.nl.fill_meta, &c.nl.fill_uv,
                      &c.nl.amap_dev};
     for (DevMem *m : all) m->release();
     if (c.blas) rocblas_destroy_handle(c.blas);
@@ -887,7 +916,9 @@ int rsdft_gpu_finalize(void)
     c.nl.nz = 0;
     c.nl.ng = 0;
     c.nl.npts = 0;
-    c.nl.ncov = 0;
+    c.nl.fill_mm = 0;
+    c.nl.fill_gmax = 0;
+    c.nl.nrange = 0;
     c.nl.color_ptr.clear();
     c.nl.natom = 0;
     c.pc_mode = 1;

@@ -16,10 +16,11 @@ constexpr int kMaxMd = 8;
 // thread, FP64 two x points tpr apart (strided); workgroup size limits
 constexpr int kHpsiPx32 = 2, kHpsiPx64 = 2;
 constexpr int kHpsiThreads32 = 1024, kHpsiThreads64 = 512;
-// FP32 H psi: nonlocal term through a compact per-covered-point buffer added
-// in the stencil epilogue (true) or scatter-added after the stencil (false).
-// Measured at Si216: 1.44 ms vs 1.26 ms per all-band H psi, so off.
-constexpr bool kFuseNonlocal = false;
+// FP32 H psi: the nonlocal term is written for every grid point of each band
+// (k_nl_fill, ranges of kFillRange points, at most kFillMaxM projector groups
+// covering a point and kFillMaxG groups touching a range) and the stencil
+// accumulates onto it
+constexpr int kFillRange = 1024, kFillMaxM = 4, kFillMaxG = 64;
 
 enum Status {
     kOK = 0,
@@ -103,12 +104,15 @@ struct NonlocalTables {
     DevMem g_val64, g_val32;    // [npts*gp] uV of each slot at each point (0 outside its sphere)
     DevMem g_order;             // groups ordered by colour
     std::vector<int> color_ptr; // colour k: g_order[color_ptr[k] .. color_ptr[k+1])
-    // covered grid points (inside some sphere) in grid order: compact index
-    // of every grid point (-1 outside), and per covered point the list of
-    // (group, union position) it belongs to
-    int ncov = 0;
-    DevMem cov_index;           // [ML]
-    DevMem mem_ptr, mem_pos, mem_g;
+    // grid-order fill of the nonlocal term (FP32 H psi): the grid cut into
+    // ranges of kFillRange points, the groups touching each range (CSR
+    // fill_rptr/fill_rgrp), and per grid point its membership count and the
+    // range-local slot of each membership (fill_meta: bits 0-2 count, 6 bits
+    // per slot from bit 3) with the uV values of membership m in
+    // fill_uv[(m*ML + i)*gp + s].  fill_mm = 0: the layout does not fit
+    // (more than kFillMaxM memberships, kFillMaxG groups per range, ...)
+    int fill_mm = 0, fill_gmax = 0, nrange = 0;
+    DevMem fill_rptr, fill_rgrp, fill_meta, fill_uv;
     DevMem scale64;             // iuV(lma) * dV
     // force tables
     int natom = 0;
@@ -169,7 +173,6 @@ struct Context {
     int wf_prec = -1;
     DevMem psi, hx, hp, gk, pgk, pk, rk, ppc, qpc, tmp;
     DevMem coefC;      // projection coefficients [lma][band]
-    DevMem nlbuf;      // nonlocal term at the covered points [band][ncov]
     DevMem part;       // reduction partial sums
     DevMem stage;      // host<->device staging (FP64)
     DevMem opbuf;      // working-precision staging of the op_* entry points
@@ -201,9 +204,10 @@ int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active = 
 template <class T>
 int nl_project(const T *in, int nb, T *coef, const int *band_active = nullptr);
 // The pieces of one H psi, for timing: the stencil kernel as hpsi_apply runs
-// it (FP32: with the compact nonlocal term of the last nl_stage call fused
-// in; FP64: kinetic + local), the nonlocal term stage (FP32: gather into the
-// compact buffer; FP64: scatter-add into out), and a plain copy of nb bands.
+// it (FP32 with the fill layout: kinetic + local accumulated onto out; else
+// kinetic + local), the nonlocal term stage (FP32 with the fill layout: the
+// term written for every grid point; else scatter-added into out), and a
+// plain copy of nb bands.
 template <class T>
 int hpsi_stencil_only(const T *in, T *out, int nb);
 template <class T>

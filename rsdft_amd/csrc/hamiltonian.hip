@@ -16,9 +16,14 @@
 //     reference's bcset_3 halo exchange (bcset_3_module.f90:15-151,
 //     bc_module.f90:722-736) has no counterpart;
 //   * the epilogue adds Vloc*psi (op_localpot);
-//   * the nonlocal term is added afterwards by k_nl_apply, which visits only
-//     the grid points inside the projector spheres, a group of projectors
-//     sharing a sphere at a time, for many bands per table load.
+//   * FP32 H psi: the nonlocal term is first written at every grid point by
+//     k_nl_fill (grid order, coefficients of each range's projector groups
+//     staged in LDS) and the stencil accumulates onto it, so the output is
+//     streamed twice instead of read-modified-written in the partial cache
+//     lines under the spheres; FP64 (and layouts the fill does not hold): the
+//     term is added afterwards by k_nl_apply, which visits only the grid
+//     points inside the projector spheres, a group of projectors sharing a
+//     sphere at a time, for many bands per table load.
 #include "common.h"
 
 #include <algorithm>
@@ -182,45 +187,92 @@ k_nl_apply(T *__restrict__ out, long ML, int nb, const int *__restrict__ order, 
 }
 
 // ---------------------------------------------------------------------------
-// Nonlocal term at the covered grid points, in compact order:
-//   nlbuf[b][c] = sum_{(g, pos) containing c} sum_s uV(pos, s) coefC[lma(g, s)][b]
-// one thread per covered point (consecutive threads write consecutive words);
-// the stencil adds it in its epilogue, so the output is written once.
+// Nonlocal term written for every grid point (FP32 H psi), in grid order:
+//   out[b][i] = sum_{groups g covering i} sum_s uV(i, g, s) coefC[lma(g, s)][b]
+// (0 outside every sphere); the stencil then accumulates kinetic + local onto
+// it, so each output element is written by two streaming passes instead of a
+// read-modify-write of the partial cache lines under the spheres.  One
+// workgroup per (range of kFillRange points, NBK bands): the coefficients of
+// the range's groups are staged in LDS ([slot][s][band], rows padded by 4
+// words so different slots start in different banks), every thread owns 4
+// consecutive points (16-byte stores per band) and reads 4 bands per LDS access.
 // ---------------------------------------------------------------------------
-template <class T, int P, int NBB>
+template <int P, int MM, int NBK>
 __global__ void __launch_bounds__(256)
-k_nl_gather(T *__restrict__ nlbuf, int ncov, int nb, int bands_per_group,
-            const int *__restrict__ mptr, const int *__restrict__ mpos, const int *__restrict__ mg,
-            const T *__restrict__ gval, const int *__restrict__ glma, const T *__restrict__ coefC,
-            const int *__restrict__ band_active)
+k_nl_fill(float *__restrict__ out, long ML, int nb, int nrange, const int *__restrict__ rptr,
+          const int *__restrict__ rgrp, const int *__restrict__ glma,
+          const float *__restrict__ coefC, const unsigned *__restrict__ meta,
+          const float *__restrict__ uv, const int *__restrict__ band_active)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncov) return;
-    const int m0 = mptr[c], m1 = mptr[c + 1];
-    const int bg0 = blockIdx.y * bands_per_group;
-    const int bg1 = min(nb, bg0 + bands_per_group);
-    for (int b0 = bg0; b0 < bg1; b0 += NBB) {
-        T acc[NBB];
+    constexpr int PT = 4, LS = P * NBK + 4;
+    static_assert(kFillRange == 256 * PT, "one point quad per thread");
+    extern __shared__ float Lc[];
+    const int nbb = (nb + NBK - 1) / NBK;
+    const long total = (long)nrange * nbb;
+    const long w = xcd_work_index(blockIdx.x, total);
+    if (w >= total) return;
+    const int r = (int)(w % nrange);
+    const int b0 = (int)(w / nrange) * NBK;
+    const int g0 = rptr[r], ngr = rptr[r + 1] - g0;
+    for (int t = threadIdx.x; t < ngr * P * NBK; t += blockDim.x) {
+        const int slot = t / (P * NBK), rem = t - slot * (P * NBK);
+        const int s = rem / NBK, k = rem - s * NBK;
+        const int lma = glma[rgrp[g0 + slot] * P + s];
+        const int b = b0 + k;
+        Lc[slot * LS + rem] = (lma >= 0 && b < nb) ? coefC[(long)lma * nb + b] : 0.0f;
+    }
+    __syncthreads();
+    const long i0 = (long)r * kFillRange + threadIdx.x * PT;
+    if (i0 >= ML) return;   // ML % 4 == 0: the quad is whole
+    const uint4 mq = *reinterpret_cast<const uint4 *>(meta + i0);
+    const unsigned mt[PT] = {mq.x, mq.y, mq.z, mq.w};
+    int cnt[PT], cmax = 0;
 #pragma unroll
-        for (int k = 0; k < NBB; ++k) acc[k] = T(0);
-        for (int m = m0; m < m1; ++m) {
-            const int g = mg[m];
-            const long pos = mpos[m];
+    for (int p = 0; p < PT; ++p) {
+        cnt[p] = (int)(mt[p] & 7u);
+        cmax = max(cmax, cnt[p]);
+    }
+    float u[MM][PT * P];
 #pragma unroll
-            for (int s = 0; s < P; ++s) {
-                const int lma = glma[g * P + s];
-                if (lma < 0) continue;
-                const T u = gval[pos * P + s];
-                const T *cr = coefC + (long)lma * nb + b0;
+    for (int m = 0; m < MM; ++m) {
+        const float4 *src = reinterpret_cast<const float4 *>(uv + ((long)m * ML + i0) * P);
 #pragma unroll
-                for (int k = 0; k < NBB; ++k)
-                    if (b0 + k < bg1) acc[k] = acc[k] + u * cr[k];
-            }
+        for (int q = 0; q < PT * P / 4; ++q) {
+            const float4 v = m < cmax ? src[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+            u[m][4 * q] = v.x;
+            u[m][4 * q + 1] = v.y;
+            u[m][4 * q + 2] = v.z;
+            u[m][4 * q + 3] = v.w;
         }
+    }
+    for (int k0 = 0; k0 < NBK; k0 += 4) {
+        float acc[PT][4];
 #pragma unroll
-        for (int k = 0; k < NBB; ++k) {
-            const int b = b0 + k;
-            if (b < bg1 && (!band_active || band_active[b])) nlbuf[(long)b * ncov + c] = acc[k];
+        for (int p = 0; p < PT; ++p)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[p][j] = 0.0f;
+#pragma unroll
+        for (int p = 0; p < PT; ++p)
+#pragma unroll
+            for (int m = 0; m < MM; ++m) {
+                if (m >= cnt[p]) continue;
+                const int base = (int)((mt[p] >> (3 + 6 * m)) & 63u) * LS + k0;
+#pragma unroll
+                for (int s = 0; s < P; ++s) {
+                    const float4 cv = *reinterpret_cast<const float4 *>(&Lc[base + s * NBK]);
+                    const float uu = u[m][p * P + s];
+                    acc[p][0] += uu * cv.x;
+                    acc[p][1] += uu * cv.y;
+                    acc[p][2] += uu * cv.z;
+                    acc[p][3] += uu * cv.w;
+                }
+            }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int b = b0 + k0 + j;
+            if (b >= nb || (band_active && !band_active[b])) continue;
+            *reinterpret_cast<float4 *>(out + (long)b * ML + i0) =
+                make_float4(acc[0][j], acc[1][j], acc[2][j], acc[3][j]);
         }
     }
 }
@@ -431,7 +483,7 @@ StencilGeom choose_hpsi_geom(int N1, int N2, int N3, int md, int px, int pad, in
 
 template <class T, int MD, int HM, int FL>
 static void launch_hpsi_k(const T *in, T *out, int nb, const int *band_active, const T *vloc,
-                          const StencilCoef<T, MD> &cf, const T *nlbuf)
+                          const StencilCoef<T, MD> &cf)
 {
     Context &c = ctx();
     const GridDesc &g = c.g;
@@ -445,13 +497,12 @@ static void launch_hpsi_k(const T *in, T *out, int nb, const int *band_active, c
     else
         hipLaunchKernelGGL((k_hpsi2<T, MD, kHpsiPx32, HM, FL, kHpsiThreads32>), grid,
                            dim3(G.threads), G.lds, c.stream, in, out, g.ML, g.N1, g.N2, g.N3, G.ty,
-                           G.tz, G.tiles, nb, cf, vloc, band_active, c.nl.cov_index.as<int>(),
-                           nlbuf, c.nl.ncov);
+                           G.tz, G.tiles, nb, cf, vloc, band_active);
 }
 
 template <class T, int MD>
 static int launch_hpsi_md(const T *in, T *out, int nb, int flags, const int *band_active,
-                          const T *vloc, const T *nlbuf)
+                          const T *vloc)
 {
     Context &c = ctx();
     const GridDesc &g = c.g;
@@ -465,17 +516,17 @@ static int launch_hpsi_md(const T *in, T *out, int nb, int flags, const int *ban
     }
     const bool h1 = G.hm == 1;
     if (flags == (kKin | kLoc)) {
-        if (h1) launch_hpsi_k<T, MD, 1, kKin | kLoc>(in, out, nb, band_active, vloc, cf, nullptr);
-        else launch_hpsi_k<T, MD, 2, kKin | kLoc>(in, out, nb, band_active, vloc, cf, nullptr);
+        if (h1) launch_hpsi_k<T, MD, 1, kKin | kLoc>(in, out, nb, band_active, vloc, cf);
+        else launch_hpsi_k<T, MD, 2, kKin | kLoc>(in, out, nb, band_active, vloc, cf);
     } else if (flags == kKin) {
-        if (h1) launch_hpsi_k<T, MD, 1, kKin>(in, out, nb, band_active, vloc, cf, nullptr);
-        else launch_hpsi_k<T, MD, 2, kKin>(in, out, nb, band_active, vloc, cf, nullptr);
-    } else if (flags == (kKin | kLoc | kNl)) {
+        if (h1) launch_hpsi_k<T, MD, 1, kKin>(in, out, nb, band_active, vloc, cf);
+        else launch_hpsi_k<T, MD, 2, kKin>(in, out, nb, band_active, vloc, cf);
+    } else if (flags == (kKin | kLoc | kAcc)) {
         if constexpr (sizeof(T) == 4) {
-            if (h1) launch_hpsi_k<T, MD, 1, kKin | kLoc | kNl>(in, out, nb, band_active, vloc, cf, nlbuf);
-            else launch_hpsi_k<T, MD, 2, kKin | kLoc | kNl>(in, out, nb, band_active, vloc, cf, nlbuf);
+            if (h1) launch_hpsi_k<T, MD, 1, kKin | kLoc | kAcc>(in, out, nb, band_active, vloc, cf);
+            else launch_hpsi_k<T, MD, 2, kKin | kLoc | kAcc>(in, out, nb, band_active, vloc, cf);
         } else {
-            return set_error(kErrArg, "fused nonlocal epilogue is FP32 only");
+            return set_error(kErrArg, "accumulating stencil is FP32 only");
         }
     } else {
         return set_error(kErrArg, "stencil flags %d not supported", flags);
@@ -500,8 +551,7 @@ k_localpot_acc(const T *__restrict__ in, T *__restrict__ out, const T *__restric
 }
 
 template <class T>
-static int launch_stencil(const T *in, T *out, int nb, int flags, const int *band_active,
-                          const T *nlbuf = nullptr)
+static int launch_stencil(const T *in, T *out, int nb, int flags, const int *band_active)
 {
     Context &c = ctx();
     const T *vloc = sizeof(T) == 8 ? (const T *)c.vloc64.p : (const T *)c.vloc32.p;
@@ -515,12 +565,12 @@ static int launch_stencil(const T *in, T *out, int nb, int flags, const int *ban
         return kOK;
     }
     switch (c.g.Md) {
-    case 1: return launch_hpsi_md<T, 1>(in, out, nb, flags, band_active, vloc, nlbuf);
-    case 2: return launch_hpsi_md<T, 2>(in, out, nb, flags, band_active, vloc, nlbuf);
-    case 3: return launch_hpsi_md<T, 3>(in, out, nb, flags, band_active, vloc, nlbuf);
-    case 4: return launch_hpsi_md<T, 4>(in, out, nb, flags, band_active, vloc, nlbuf);
-    case 6: return launch_hpsi_md<T, 6>(in, out, nb, flags, band_active, vloc, nlbuf);
-    case 8: return launch_hpsi_md<T, 8>(in, out, nb, flags, band_active, vloc, nlbuf);
+    case 1: return launch_hpsi_md<T, 1>(in, out, nb, flags, band_active, vloc);
+    case 2: return launch_hpsi_md<T, 2>(in, out, nb, flags, band_active, vloc);
+    case 3: return launch_hpsi_md<T, 3>(in, out, nb, flags, band_active, vloc);
+    case 4: return launch_hpsi_md<T, 4>(in, out, nb, flags, band_active, vloc);
+    case 6: return launch_hpsi_md<T, 6>(in, out, nb, flags, band_active, vloc);
+    case 8: return launch_hpsi_md<T, 8>(in, out, nb, flags, band_active, vloc);
     default: return set_error(kErrArg, "Md=%d not compiled (1,2,3,4,6,8)", c.g.Md);
     }
 }
@@ -592,38 +642,40 @@ int nl_apply(T *out, int nb, const T *coef, const int *band_active)
     return kOK;
 }
 
-template <class T, int P>
-static void launch_nl_gather(int nb, const T *coef, const int *band_active, T *nlbuf)
+template <int P, int MM>
+static void launch_nl_fill(int nb, const float *coef, const int *band_active, float *out)
 {
     Context &c = ctx();
-    NonlocalTables &nl = c.nl;
-    constexpr int NBB = 16;
-    const int nblk = (nl.ncov + 255) / 256;
-    int ngroups = std::max(1, std::min((nb + NBB - 1) / NBB, (4096 + nblk - 1) / nblk));
-    int bpg = (nb + ngroups - 1) / ngroups;
-    bpg = (bpg + NBB - 1) / NBB * NBB;
-    ngroups = (nb + bpg - 1) / bpg;
-    const T *val = sizeof(T) == 8 ? (const T *)nl.g_val64.p : (const T *)nl.g_val32.p;
-    hipLaunchKernelGGL((k_nl_gather<T, P, NBB>), dim3(nblk, ngroups), dim3(256), 0, c.stream, nlbuf,
-                       nl.ncov, nb, bpg, nl.mem_ptr.as<int>(), nl.mem_pos.as<int>(),
-                       nl.mem_g.as<int>(), val, nl.g_lma.as<int>(), coef, band_active);
+    const NonlocalTables &nl = c.nl;
+    constexpr int NBK = P <= 4 ? 32 : 128 / P;
+    constexpr int LS = P * NBK + 4;
+    const long nblk = (long)nl.nrange * ((nb + NBK - 1) / NBK);
+    const size_t lds = sizeof(float) * (size_t)std::max(nl.fill_gmax, 1) * LS;
+    hipLaunchKernelGGL((k_nl_fill<P, MM, NBK>), dim3((unsigned)((nblk + 7) / 8 * 8)), dim3(256),
+                       lds, c.stream, out, c.g.ML, nb, nl.nrange, nl.fill_rptr.as<int>(),
+                       nl.fill_rgrp.as<int>(), nl.g_lma.as<int>(), coef,
+                       nl.fill_meta.as<unsigned>(), nl.fill_uv.as<float>(), band_active);
 }
 
-template <class T>
-static int nl_gather(int nb, const T *coef, const int *band_active, T *nlbuf)
+// out[b][i] <- nonlocal term at every grid point of the active bands
+static int nl_fill(int nb, const float *coef, const int *band_active, float *out)
 {
-    Context &c = ctx();
-    switch (c.nl.gp) {
-    case 1: launch_nl_gather<T, 1>(nb, coef, band_active, nlbuf); break;
-    case 2: launch_nl_gather<T, 2>(nb, coef, band_active, nlbuf); break;
-    case 4: launch_nl_gather<T, 4>(nb, coef, band_active, nlbuf); break;
-    case 8: launch_nl_gather<T, 8>(nb, coef, band_active, nlbuf); break;
-    case 16: launch_nl_gather<T, 16>(nb, coef, band_active, nlbuf); break;
-    default: return set_error(kErrState, "projector group width %d", c.nl.gp);
+    const NonlocalTables &nl = ctx().nl;
+    switch (nl.gp * 8 + nl.fill_mm) {
+#define RS_FILL_CASE(P, MM) \
+    case P * 8 + MM: launch_nl_fill<P, MM>(nb, coef, band_active, out); break;
+    RS_FILL_CASE(1, 1) RS_FILL_CASE(1, 2) RS_FILL_CASE(1, 4)
+    RS_FILL_CASE(2, 1) RS_FILL_CASE(2, 2) RS_FILL_CASE(2, 4)
+    RS_FILL_CASE(4, 1) RS_FILL_CASE(4, 2) RS_FILL_CASE(4, 4)
+    RS_FILL_CASE(8, 1) RS_FILL_CASE(8, 2) RS_FILL_CASE(16, 1)
+#undef RS_FILL_CASE
+    default: return set_error(kErrState, "fill layout gp=%d mm=%d", nl.gp, nl.fill_mm);
     }
     RS_HIP(hipGetLastError());
     return kOK;
 }
+
+static bool fill_ready() { return ctx().nl.nz > 0 && ctx().nl.fill_mm > 0; }
 
 template <class T>
 int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
@@ -636,15 +688,15 @@ int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
         RS_TRY(c.coefC.ensure((size_t)nb * c.nl.nz * sizeof(T)));
         RS_TRY(nl_project<T>(in, nb, c.coefC.as<T>(), band_active));
     }
-    // FP32 H psi with kFuseNonlocal: the nonlocal term goes through the
-    // compact buffer into the stencil epilogue (one write of the output)
-    if (kFuseNonlocal && sizeof(T) == 4 && sflags == (kKin | kLoc) && (flags & kNl) &&
-        c.nl.ncov > 0) {
-        RS_TRY(c.nlbuf.ensure((size_t)nb * c.nl.ncov * sizeof(T)));
-        RS_TRY(nl_gather<T>(nb, c.coefC.as<T>(), band_active, c.nlbuf.as<T>()));
-        RS_TRY(launch_stencil<T>(in, out, nb, kKin | kLoc | kNl, band_active, c.nlbuf.as<T>()));
-        c.hpsi_bands += nb;
-        return kOK;
+    if constexpr (sizeof(T) == 4) {
+        // FP32 H psi: the nonlocal term written for every grid point, then the
+        // stencil accumulates kinetic + local onto it
+        if (sflags == (kKin | kLoc) && (flags & kNl) && fill_ready()) {
+            RS_TRY(nl_fill(nb, c.coefC.as<float>(), band_active, out));
+            RS_TRY(launch_stencil<T>(in, out, nb, kKin | kLoc | kAcc, band_active));
+            c.hpsi_bands += nb;
+            return kOK;
+        }
     }
     if (sflags & (kKin | kLoc)) {
         RS_TRY(launch_stencil<T>(in, out, nb, sflags, band_active));
@@ -659,22 +711,16 @@ int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
 template <class T>
 int hpsi_stencil_only(const T *in, T *out, int nb)
 {
-    Context &c = ctx();
-    if (kFuseNonlocal && sizeof(T) == 4 && c.nl.nz > 0 && c.nl.ncov > 0) {
-        RS_TRY(c.nlbuf.ensure((size_t)nb * c.nl.ncov * sizeof(T)));
-        return launch_stencil<T>(in, out, nb, kKin | kLoc | kNl, nullptr, c.nlbuf.as<T>());
-    }
+    if constexpr (sizeof(T) == 4)
+        if (fill_ready()) return launch_stencil<T>(in, out, nb, kKin | kLoc | kAcc, nullptr);
     return launch_stencil<T>(in, out, nb, kKin | kLoc, nullptr);
 }
 
 template <class T>
 int nl_stage(T *out, int nb, const T *coef)
 {
-    Context &c = ctx();
-    if (kFuseNonlocal && sizeof(T) == 4 && c.nl.nz > 0 && c.nl.ncov > 0) {
-        RS_TRY(c.nlbuf.ensure((size_t)nb * c.nl.ncov * sizeof(T)));
-        return nl_gather<T>(nb, coef, nullptr, c.nlbuf.as<T>());
-    }
+    if constexpr (sizeof(T) == 4)
+        if (fill_ready()) return nl_fill(nb, coef, nullptr, out);
     return nl_apply<T>(out, nb, coef, nullptr);
 }
 

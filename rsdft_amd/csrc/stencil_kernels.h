@@ -34,17 +34,6 @@ constexpr int stencil_pad() { return (MD + PX - 1) / PX * PX; }
 constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
 constexpr int clcm(int a, int b) { return a / cgcd(a, b) * b; }
 
-// Prefetch distance of the covered-point index stream of the fused nonlocal
-// epilogue: the index of plane p is requested D2 planes ahead and the
-// nonlocal value PF planes ahead, so D2 > PF; D2 + 1 divides the queue length.
-template <int QD, int PF>
-constexpr int stencil_d2()
-{
-    for (int d = PF + 2; d <= QD; ++d)
-        if (QD % d == 0) return d - 1;
-    return QD - 1;
-}
-
 template <class T, int PX>
 struct alignas(sizeof(T) * PX) Chunk {
     T v[PX];
@@ -67,8 +56,7 @@ template <class T, int MD, int PX, int HM, int FL, int NT = 512, int HE = 0>
 __global__ void __launch_bounds__(NT)
 k_hpsi2(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, int N3, int ty,
         int tz, int tiles, int nbands, StencilCoef<T, MD> cf, const T *__restrict__ vloc,
-        const int *__restrict__ band_active, const int *__restrict__ cov_index,
-        const T *__restrict__ nlbuf, int ncov)
+        const int *__restrict__ band_active)
 {
     using C = Chunk<T, PX>;
     constexpr int PF = stencil_pf<MD>();
@@ -84,10 +72,6 @@ k_hpsi2(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, 
     constexpr int HR = HD + 1;
     constexpr int UL = clcm(QD, HR);
     constexpr bool KIN = (FL & kKin) != 0, LOC = (FL & kLoc) != 0, ACC = (FL & kAcc) != 0;
-    // NLA: add the nonlocal term nlbuf[band][cov_index[i]] at covered points
-    constexpr bool NLA = (FL & kNl) != 0;
-    constexpr int D2 = stencil_d2<2 * MD + 1 + stencil_pf<MD>(), stencil_pf<MD>()>();
-    constexpr int R2 = D2 + 1;
     extern __shared__ unsigned char smem_raw[];
     C *L = reinterpret_cast<C *>(smem_raw);
     const int tpr = N1 / PX;
@@ -127,11 +111,6 @@ k_hpsi2(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, 
     }
 
     C q[QD], hb[HR][HM], vq[RG], dq[RG];
-    using CI = Chunk<int, PX>;
-    CI cv[NLA ? R2 : 1];
-    C nv[NLA ? RG : 1];
-    const CI *cisrc = reinterpret_cast<const CI *>(cov_index);
-    const T *nlband = NLA ? nlbuf + (long)band * ncov : nullptr;
 #pragma unroll
     for (int k = 0; k < QD - 1; ++k) q[k] = src[wrapi(z0 - MD + k, N3) * planeC + rowC];
 #pragma unroll
@@ -146,17 +125,6 @@ k_hpsi2(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, 
         const long zp = wrapi(z0 + k, N3) * planeC;
         if (LOC) vq[k] = vsrc[zp + rowC];
         if (ACC && k < tz) dq[k] = dst[zp + rowC];
-    }
-    if constexpr (NLA) {
-#pragma unroll
-        for (int k = 0; k < D2; ++k) cv[k % R2] = cisrc[wrapi(z0 + k, N3) * planeC + rowC];
-#pragma unroll
-        for (int k = 0; k < PF; ++k)
-#pragma unroll
-            for (int px = 0; px < PX; ++px) {
-                const int ci = cv[k % R2].v[px];
-                nv[k % RG].v[px] = ci >= 0 ? nlband[ci] : T(0);
-            }
     }
     const int lrow = (r + MD) * WC + NC + c;
     for (int zb = z0; zb < z0 + tz; zb += UL) {
@@ -176,16 +144,6 @@ k_hpsi2(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, 
                     if (h_on[h]) hb[(u + HD) % HR][h] = src[zh * planeC + h_g[h]];
                 if (LOC) vq[(u + PF) % RG] = vsrc[zv * planeC + rowC];
                 if (ACC && z + PF < z0 + tz) dq[(u + PF) % RG] = dst[zv * planeC + rowC];
-                if constexpr (NLA) {
-                    int zc = z + D2;
-                    while (zc >= N3) zc -= N3;
-                    cv[(u + D2) % R2] = cisrc[zc * planeC + rowC];
-#pragma unroll
-                    for (int px = 0; px < PX; ++px) {
-                        const int ci = cv[(u + PF) % R2].v[px];
-                        nv[(u + PF) % RG].v[px] = ci >= 0 ? nlband[ci] : T(0);
-                    }
-                }
             }
             const C cen = q[(u + MD) % QD];
             Lb[lrow] = cen;
@@ -235,7 +193,6 @@ k_hpsi2(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, 
 #pragma unroll
             for (int px = 0; px < PX; ++px) {
                 if (LOC) o.v[px] = o.v[px] + vq[u % RG].v[px] * cen.v[px];
-                if constexpr (NLA) o.v[px] = o.v[px] + nv[u % RG].v[px];
                 if (ACC) o.v[px] = dq[u % RG].v[px] + o.v[px];
             }
             dst[(long)z * planeC + rowC] = o;
