@@ -72,10 +72,13 @@ RSDFT_API int rsdft_gpu_set_precond_mode(int mode);
 
 /* Eigensolver of the subspace_diag step (subspace_diag_la_module.f90:159-169):
  * 1 (default) quadratically convergent refinement from the identity when
- * Hsub is nearly diagonal, otherwise mode 2; 2 -- Householder reduction to
- * tridiagonal form in one cooperative launch with Hsub resident in LDS, then
- * rocsolver_dstedc and rocsolver_dormtr (rocsolver_dsyevd when Hsub does not
- * fit the LDS of the device); 0 -- always rocsolver_dsyevd. */
+ * Hsub is nearly diagonal, otherwise mode 3; 3 -- Householder reduction to
+ * tridiagonal form in one cooperative launch with Hsub resident in LDS, the
+ * tridiagonal eigenpairs by multisection on Sturm counts + inverse iteration
+ * (Gram-Schmidt inside clusters, one Newton-Schulz orthogonalisation step;
+ * rocsolver_dstedc above n = 512), then the blocked back-transformation
+ * (rocsolver_dormtr above n = 512); 2 -- the same with rocsolver_dstedc always;
+ * 0 -- always rocsolver_dsyevd (also when Hsub does not fit the LDS). */
 RSDFT_API int rsdft_gpu_set_eigensolver(int mode);
 
 /* How rsdft_gpu_sweep runs gram_schmidt + subspace_diag after CG

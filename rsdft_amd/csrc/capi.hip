@@ -535,7 +535,7 @@ int rsdft_gpu_set_precond_mode(int mode)
 
 int rsdft_gpu_set_eigensolver(int mode)
 {
-    if (mode < 0 || mode > 2) return set_error(kErrArg, "eigensolver mode %d", mode);
+    if (mode < 0 || mode > 3) return set_error(kErrArg, "eigensolver mode %d", mode);
     ctx().eig_mode = mode;
     return kOK;
 }
@@ -840,7 +840,7 @@ int rsdft_gpu_test_hpsi(int nrep, double *times_ms)
 int rsdft_gpu_test_eigh(const double *H, int n, int mode, double *w, double *V, double *ms)
 {
     RS_TRY(need_init());
-    if (!H || !w || n < 1 || mode < 0 || mode > 2)
+    if (!H || !w || n < 1 || mode < 0 || mode > 3)
         return set_error(kErrArg, "test_eigh: n=%d mode=%d", n, mode);
     Context &c = ctx();
     const size_t nn = (size_t)n * n;

@@ -160,7 +160,7 @@ struct Context {
     int ncg = 3, mloop = 3;
     int pc_mode = 1;   // 0: step-by-step preconditioner recurrence, 1: s-step form
     size_t max_lds = 64 * 1024;   // LDS bytes one workgroup may allocate on this device
-    int eig_mode = 1;  // 0: dsyevd, 1: refinement when Hsub is nearly diagonal, else 2: own tridiagonal reduction
+    int eig_mode = 1;  // 0: dsyevd, 1: refinement when Hsub is nearly diagonal, else 3; 2: own tridiagonal reduction + dstedc; 3: own reduction + bisection / inverse iteration
     int sweep_mode = 1;  // 1: Rayleigh-Ritz through the Cholesky factor and CG's H x, 0: GS then subspace_diag
     long eig_refined = 0;
 

@@ -17,7 +17,8 @@
 
 namespace rsdft {
 
-int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *done);
+int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *done,
+                 int tri_solver);
 template <class T>
 int split_gemm_tn(const float *A, const float *B, int nb, long K, double alpha, double *C,
                   int sym_result);
@@ -565,8 +566,9 @@ static int chol_inverse(double *S, int nb)
 
 // Eigenpairs of the symmetric Hsub H (destroyed): eigenvalues ascending in D,
 // eigenvectors (columns, V(j,j) >= 0) at *Vec, which is H or Vbuf.
-// eig_mode 1: refinement when H is nearly diagonal, else the LDS-resident
-// tridiagonal reduction with rocsolver_dstedc / dormtr; 2: the latter always;
+// eig_mode 1: refinement when H is nearly diagonal, else as mode 3;
+// 2: the LDS-resident tridiagonal reduction with rocsolver_dstedc;
+// 3: the reduction with bisection + inverse iteration on the tridiagonal;
 // 0 (and matrices too large for the reduction): rocsolver_dsyevd.
 int eigh_sub(double *H, int nb, double *Vbuf, double *D, double **Vec)
 {
@@ -579,7 +581,8 @@ int eigh_sub(double *H, int nb, double *Vbuf, double *D, double **Vec)
         c.eig_refined += refined;
         done = refined;
     }
-    if (!done && c.eig_mode != 0) RS_TRY(eigh_tridiag(H, nb, Vbuf, D, info, &done));
+    if (!done && c.eig_mode != 0)
+        RS_TRY(eigh_tridiag(H, nb, Vbuf, D, info, &done, c.eig_mode == 2 ? 0 : 1));
     if (done) {
         *Vec = Vbuf;
     } else {

@@ -1,7 +1,7 @@
 // Symmetric eigensolver of the subspace matrix: Householder reduction to
-// tridiagonal form in one cooperative launch, rocsolver_dstedc on the
-// tridiagonal, and the back-transformation by a blocked wave-per-columns
-// kernel.
+// tridiagonal form in one cooperative launch, the tridiagonal eigenproblem by
+// bisection + inverse iteration (or rocsolver_dstedc), and the
+// back-transformation by a blocked wave-per-columns kernel.
 //
 // The reduction is dsytd2 with the lower convention: A = Q T Q^T,
 // Q = H(0) H(1) ... H(n-2), H(i) = I - tau_i v_i v_i^T with v_i(0:i) = 0,
@@ -325,6 +325,322 @@ __global__ void k_trd_flag(const int *err, const int *info, int *flag)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Eigenpairs of the tridiagonal T = tridiag(e, d, e), n <= kTriMax, in place of
+// rocsolver_dstedc (whose secular-equation merges take ~2 ms at n = 480):
+//  1. eigenvalues: one wave per index j, multisection on Sturm counts (the
+//     number of negative pivots of T - sigma I by LAPACK dstebz's recurrence
+//     q_i = (d_i - sigma) - e_{i-1}^2 / q_{i-1} with its pivmin guard, the
+//     quotient from a Newton-refined hardware reciprocal); each round the 64
+//     lanes evaluate kBisS shifts each (independent recurrences, so their
+//     latencies overlap), cutting [lo, hi] into 64 kBisS + 1 parts, until
+//     hi - lo <= 2 eps max(|lo|, |hi|) + eps |T| / 2;
+//  2. eigenvectors: inverse iteration as in LAPACK dstein: LU with partial
+//     pivoting of T - lambda_j I (dgttrf, pivots below eps |T| raised to it),
+//     kInvRounds solves (dgttrs) from a fixed pseudo-random start, each
+//     followed by modified Gram-Schmidt inside clusters of eigenvalues closer
+//     than kClusterTol |T| (inverse iteration alone cannot separate those;
+//     a second round makes the start vector's component along the wanted
+//     eigenvector irrelevant);
+//  3. one Newton-Schulz step Z <- Z (3I - Z^T Z) / 2 removes the remaining
+//     O(eps |T| / gap) non-orthogonality between vectors of different
+//     clusters; it moves vector j by sum_k O(eps |T| / gap_jk) z_k, which
+//     changes its residual by O(eps |T|).
+// |T| is the Gershgorin bound max(|gl|, |gu|).
+// ---------------------------------------------------------------------------
+constexpr int kTriMax = 512;
+constexpr int kBisS = 2;
+constexpr int kInvRounds = 2;
+constexpr double kClusterTol = 1e-7;
+constexpr double kEps = 1.1102230246251565e-16;   // 2^-53
+constexpr double kSafeMin = 2.2250738585072014e-308;
+
+// 1/q: the hardware reciprocal refined by one Newton step (which doubles the
+// correct bits); |q| >= pivmin keeps it finite.  A Sturm count from these
+// quotients is the exact count of T with each d_i perturbed relatively by
+// O(eps), as with IEEE division.
+__device__ __forceinline__ double tri_recip(double q)
+{
+    const double r = __builtin_amdgcn_rcp(q);
+    return fma(r, fma(-q, r, 1.0), r);
+}
+
+__global__ void __launch_bounds__(64)
+k_tri_bisect(const double *__restrict__ d, const double *__restrict__ e, int n,
+             double *__restrict__ w, double *__restrict__ tnorm_out, int *__restrict__ info)
+{
+    __shared__ double sd[kTriMax], se2[kTriMax];
+    const int j = blockIdx.x, lane = threadIdx.x;
+    double gl = INFINITY, gu = -INFINITY, emax2 = 0.0;
+    for (int i = lane; i < n; i += 64) {
+        const double el = i > 0 ? fabs(e[i - 1]) : 0.0, er = i < n - 1 ? fabs(e[i]) : 0.0;
+        sd[i] = d[i];
+        se2[i] = er * er;
+        gl = fmin(gl, d[i] - el - er);
+        gu = fmax(gu, d[i] + el + er);
+        emax2 = fmax(emax2, er * er);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        gl = fmin(gl, __shfl_xor(gl, o, 64));
+        gu = fmax(gu, __shfl_xor(gu, o, 64));
+        emax2 = fmax(emax2, __shfl_xor(emax2, o, 64));
+    }
+    __syncthreads();
+    const double tnorm = fmax(fabs(gl), fabs(gu));
+    const double pivmin = kSafeMin * fmax(1.0, emax2);
+    double lo = gl - 2.0 * kEps * tnorm - 2.0 * pivmin;
+    double hi = gu + 2.0 * kEps * tnorm + 2.0 * pivmin;
+    constexpr int K = 64 * kBisS;
+    // invariant: count(lo) <= j < count(hi), count(s) = #eigenvalues below s
+    for (int round = 0; round < 64; ++round) {
+        if (hi - lo <= 2.0 * kEps * fmax(fabs(lo), fabs(hi)) + 0.5 * kEps * tnorm) break;
+        const double h = (hi - lo) / (double)(K + 1);
+        double q[kBisS], sg[kBisS];
+        int cnt[kBisS];
+#pragma unroll
+        for (int s = 0; s < kBisS; ++s) {
+            sg[s] = lo + (double)(lane * kBisS + s + 1) * h;
+            double t = sd[0] - sg[s];
+            if (fabs(t) < pivmin) t = -pivmin;
+            q[s] = t;
+            cnt[s] = t < 0.0 ? 1 : 0;
+        }
+#pragma unroll 8
+        for (int i = 1; i < n; ++i) {
+            const double di = sd[i], e2 = se2[i - 1];
+#pragma unroll
+            for (int s = 0; s < kBisS; ++s) {
+                double t = (di - sg[s]) - e2 * tri_recip(q[s]);
+                if (fabs(t) < pivmin) t = -pivmin;
+                q[s] = t;
+                cnt[s] += t < 0.0 ? 1 : 0;
+            }
+        }
+        // first shift (in increasing order) with count >= j + 1
+        int kl = K;
+#pragma unroll
+        for (int s = kBisS - 1; s >= 0; --s)
+            if (cnt[s] >= j + 1) kl = lane * kBisS + s;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) kl = min(kl, __shfl_xor(kl, o, 64));
+        const double nlo = lo + (double)kl * h;                      // shift kl - 1 (or lo)
+        const double nhi = kl < K ? lo + (double)(kl + 1) * h : hi;  // shift kl (or hi)
+        lo = nlo;
+        hi = nhi;
+    }
+    if (lane == 0) {
+        w[j] = 0.5 * (lo + hi);
+        if (j == 0) {
+            *tnorm_out = tnorm;
+            *info = 0;
+        }
+    }
+}
+
+__device__ __forceinline__ double tri_start(int i, int j)
+{
+    // fixed pseudo-random start vector in [-1, 1) (a 32-bit integer mix)
+    unsigned h = (unsigned)i * 0x9E3779B1u ^ ((unsigned)j + 0x7F4A7C15u) * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return (double)h * (2.0 / 4294967296.0) - 1.0;
+}
+
+// X(:, j) <- the fixed pseudo-random start vector, S[j] = 1
+__global__ void k_tri_start(double *__restrict__ X, double *__restrict__ S, int n)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= n) return;
+    X[(long)i * n + j] = tri_start(i, j);
+    if (i == 0) S[j] = 1.0;
+}
+
+// one inverse-iteration solve per eigenvector, lane j of the grid owning
+// eigenvector j: X(i, j) = X[i*n + j] (consecutive lanes touch consecutive
+// words).  Round 0 first factors T - w_j I (dgttrf; the multipliers,
+// reciprocal pivots, first and second superdiagonals and row-interchange
+// flags stored in the same layout).  Every round solves with
+// Xin(:, j) * S[j] into Xout and sets S[j] = 1 / |Xout(:, j)|.  The
+// recurrences carry their values in registers, branches are selects, and
+// input, temporary and output are distinct arrays, so the loads of a sweep
+// are issued ahead of the dependency chain.
+__global__ void __launch_bounds__(64)
+k_tri_invit(const double *__restrict__ d, const double *__restrict__ e,
+            const double *__restrict__ w, const double *__restrict__ tnorm_p, int n, int round,
+            double *__restrict__ FL, double *__restrict__ FR, double *__restrict__ FU,
+            double *__restrict__ FU2, unsigned char *__restrict__ FP,
+            const double *__restrict__ Xin, double *__restrict__ Y, double *__restrict__ Xout,
+            double *__restrict__ S)
+{
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    if (j >= n) return;
+    const long N = n;
+    if (round == 0) {
+        const double lam = w[j];
+        const double ptol = fmax(kEps * *tnorm_p, kSafeMin);
+        double dc = d[0] - lam, duc = n > 1 ? e[0] : 0.0;   // current D(i), DU(i)
+#pragma unroll 4
+        for (int i = 0; i < n - 1; ++i) {
+            const double li = e[i], dn = d[i + 1] - lam;
+            const double dun = i < n - 2 ? e[i + 1] : 0.0;
+            const bool keep = fabs(dc) >= fabs(li);   // no row interchange
+            const double num = keep ? li : dc;
+            double den = keep ? dc : li;
+            den = den == 0.0 ? 1.0 : den;             // keep with dc = 0 means li = 0
+            const double f = num * tri_recip(den);
+            double piv = keep ? dc : li;
+            const double ndc = keep ? dn - f * duc : duc - f * dn;
+            const double nduc = keep ? dun : -f * dun;
+            if (fabs(piv) < ptol) piv = piv < 0.0 ? -ptol : ptol;
+            const long ix = (long)i * N + j;
+            FL[ix] = f;
+            FR[ix] = tri_recip(piv);
+            FU[ix] = keep ? duc : dn;
+            FU2[ix] = keep ? 0.0 : dun;
+            FP[ix] = keep ? 0 : 1;
+            dc = ndc;
+            duc = nduc;
+        }
+        if (fabs(dc) < ptol) dc = dc < 0.0 ? -ptol : ptol;
+        FR[(long)(n - 1) * N + j] = tri_recip(dc);
+    }
+    // P and L: Y = L^-1 P b
+    const double sin = S[j];
+    double bc = Xin[j] * sin;
+#pragma unroll 8
+    for (int i = 0; i < n - 1; ++i) {
+        const long ix = (long)i * N + j;
+        const double bn = Xin[ix + N] * sin;
+        const double f = FL[ix];
+        const bool sw = FP[ix] != 0;
+        const double y = sw ? bn : bc, o = sw ? bc : bn;
+        Y[ix] = y;
+        bc = o - f * y;
+    }
+    Y[(long)(n - 1) * N + j] = bc;
+    // U: Xout = U^-1 Y with the sum of squares.  A block of 8 steps grows |x|
+    // by at most (2 |T| / ptol)^8 < 1e132, so rescaling by 1e-100 after every
+    // block in which |x| passed 1e100 keeps |x| below 1e232.  The block's
+    // squares are summed as (x 1e-100)^2 (<= 1e264, finite): that is the
+    // square of the rescaled value, or 1e-200 x^2 without rescale.
+    double x1 = 0.0, x2 = 0.0, ysc = 1.0, ss = 0.0;   // x(i+1), x(i+2), scale of the rest of Y
+    int i = n - 1;
+    while (i >= 0) {
+        const int i1 = max(i - 7, 0);
+        double mx = 0.0, sb = 0.0;
+#pragma unroll 8
+        for (int k = 0; k < 8; ++k) {
+            const int ii = i - k;
+            if (ii < i1) break;
+            const long ix = (long)ii * N + j;
+            const double du = ii < n - 1 ? FU[ix] : 0.0;
+            const double du2 = ii < n - 2 ? FU2[ix] : 0.0;
+            const double x = (Y[ix] * ysc - du * x1 - du2 * x2) * FR[ix];
+            Xout[ix] = x;
+            mx = fmax(mx, fabs(x));
+            sb += (x * 1e-100) * (x * 1e-100);
+            x2 = x1;
+            x1 = x;
+        }
+        if (mx > 1e100) {   // rescale the computed tail (rare)
+            for (int k = i1; k < n; ++k) Xout[(long)k * N + j] *= 1e-100;
+            x1 *= 1e-100;
+            x2 *= 1e-100;
+            ysc *= 1e-100;
+            ss = ss * 1e-200 + sb;
+        } else {
+            ss += sb * 1e200;   // sb <= 8 here
+        }
+        i = i1 - 1;
+    }
+    S[j] = 1.0 / sqrt(ss);
+}
+
+// Xout(:, j) *= S[j] for the members of clusters (the others keep the factor),
+// then modified Gram-Schmidt inside each cluster (consecutive eigenvalues
+// closer than kClusterTol |T|): one wave per cluster, started by its first
+// member; members end with S = 1
+__global__ void __launch_bounds__(64)
+k_tri_mgs(const double *__restrict__ w, const double *__restrict__ tnorm_p, int n,
+          double *__restrict__ X, double *__restrict__ S)
+{
+    const int j = blockIdx.x, lane = threadIdx.x;
+    const long N = n;
+    const double ctol = kClusterTol * *tnorm_p;
+    if (j + 1 >= n || w[j + 1] - w[j] >= ctol) return;
+    if (j > 0 && w[j] - w[j - 1] < ctol) return;
+    int m = j + 1;
+    while (m + 1 < n && w[m + 1] - w[m] < ctol) ++m;
+    for (int k = j; k <= m; ++k) {
+        const double sk = S[k];
+        for (int i = lane; i < n; i += 64) X[i * N + k] *= sk;
+    }
+    for (int k = j + 1; k <= m; ++k) {
+        for (int i0 = j; i0 < k; ++i0) {
+            double s = 0.0;
+            for (int i = lane; i < n; i += 64) s += X[i * N + i0] * X[i * N + k];
+            s = coop_wave_sum(s);
+            for (int i = lane; i < n; i += 64) X[i * N + k] -= s * X[i * N + i0];
+        }
+        double s = 0.0;
+        for (int i = lane; i < n; i += 64) s += X[i * N + k] * X[i * N + k];
+        const double inv = 1.0 / sqrt(coop_wave_sum(s));
+        for (int i = lane; i < n; i += 64) X[i * N + k] *= inv;
+    }
+    __syncthreads();   // every lane has read S before it is reset
+    for (int k = j + lane; k <= m; k += 64) S[k] = 1.0;
+}
+
+// X(:, j) *= S[j]
+__global__ void k_tri_scale(double *__restrict__ X, const double *__restrict__ S, int n)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j < n) X[(long)i * n + j] *= S[j];
+}
+
+__global__ void k_tri_identity(double *G, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i < n) G[(long)j * n + i] = i == j ? 1.0 : 0.0;
+}
+
+// eigenvalues of T into D (ascending, replacing the diagonal) and
+// eigenvectors into V; scratch: W (2n + 8), X (3 n x n: two iterates and the
+// forward-solve temporary), G and the four factor arrays (n x n each), FP
+// (n x n bytes)
+static int tri_eig(int n, double *D, const double *E, double *V, int *einfo, double *W,
+                   double *X, double *G, double *F, unsigned char *FP)
+{
+    Context &c = ctx();
+    const size_t nn = (size_t)n * n;
+    double *tn = W + n, *S = W + n + 8;
+    hipLaunchKernelGGL(k_tri_bisect, dim3(n), dim3(64), 0, c.stream, D, E, n, W, tn, einfo);
+    const dim3 ig((n + 63) / 64);
+    double *Xa = X, *Xb = X + nn, *Y = X + 2 * nn;
+    hipLaunchKernelGGL(k_tri_start, dim3((n + 127) / 128, n), dim3(128), 0, c.stream, Xa, S, n);
+    for (int r = 0; r < kInvRounds; ++r) {
+        hipLaunchKernelGGL(k_tri_invit, ig, dim3(64), 0, c.stream, D, E, W, tn, n, r, F, F + nn,
+                           F + 2 * nn, F + 3 * nn, FP, Xa, Y, Xb, S);
+        hipLaunchKernelGGL(k_tri_mgs, dim3(n), dim3(64), 0, c.stream, W, tn, n, Xb, S);
+        std::swap(Xa, Xb);
+    }
+    hipLaunchKernelGGL(k_tri_scale, dim3((n + 127) / 128, n), dim3(128), 0, c.stream, Xa, S, n);
+    hipLaunchKernelGGL(k_tri_identity, dim3((n + 127) / 128, n), dim3(128), 0, c.stream, G, n);
+    RS_HIP(hipGetLastError());
+    // Xa holds Z^T column-major: G = 1.5 I - 0.5 Z^T Z = 1.5 I - 0.5 Xa Xa^T, V = Z G = Xa^T G
+    const double mh = -0.5, th = 1.5, one = 1.0, zero = 0.0;
+    RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_transpose, n, n, n,
+                          &mh, Xa, n, Xa, n, &th, G, n));
+    RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, n, n, n,
+                          &one, Xa, n, G, n, &zero, V, n));
+    RS_HIP(hipMemcpyAsync(D, W, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+    return kOK;
+}
+
 // workgroups and LDS bytes of the cooperative reduction (about 4 columns per
 // workgroup), false if n does not fit
 static bool trd_shape(int n, int *nwg, int *cpw, size_t *lds)
@@ -347,10 +663,13 @@ static bool trd_shape(int n, int *nwg, int *cpw, size_t *lds)
 }
 
 // Eigenpairs of the symmetric H (n x n, full storage): eigenvalues ascending
-// in D, eigenvectors in V.  einfo[0] receives the dstedc info and einfo[3] a
-// nonzero flag on failure.  Returns kOK with *done = 0 when the matrix is too
-// large for the LDS-resident reduction (nothing launched then).
-int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *done)
+// in D, eigenvectors in V.  The tridiagonal eigenproblem goes to tri_eig
+// (tri_solver 1, n <= kTriMax) or rocsolver_dstedc.  einfo[0] receives the
+// tridiagonal solver's info and einfo[3] a nonzero flag on failure.  Returns
+// kOK with *done = 0 when the matrix is too large for the LDS-resident
+// reduction (nothing launched then).
+int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *done,
+                 int tri_solver)
 {
     Context &c = ctx();
     *done = 0;
@@ -361,11 +680,16 @@ int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *
     const int nflags = 16 * nwg;
     const int nqb = (n - 1 + kQBlk - 1) / kQBlk;
     const size_t nc = (size_t)nqb * kQBlk * kQBlk;
-    RS_TRY(c.evwork.ensure(sizeof(double) * (nn + 6 * (size_t)n + nc) +
+    const bool own = tri_solver == 1 && n <= kTriMax;
+    // tri_eig scratch: W, 3 X, G, 4 factor arrays, the interchange flags (bytes)
+    const size_t nown = own ? 8 * nn + 2 * (size_t)n + 8 + (nn + 7) / 8 : 0;
+    RS_TRY(c.evwork.ensure(sizeof(double) * (nn + 6 * (size_t)n + nc + nown) +
                            sizeof(int) * (nflags + 4)));
     double *R = c.evwork.as<double>(), *E = R + nn, *TAU = E + n, *pub = TAU + n;
     double *Cq = pub + 4 * (size_t)n;
-    unsigned *flags = reinterpret_cast<unsigned *>(Cq + nc);
+    double *Wt = Cq + nc, *Xt = Wt + 2 * n + 8, *Gt = Xt + 3 * nn, *Ft = Gt + nn;
+    unsigned char *FPt = reinterpret_cast<unsigned char *>(Ft + 4 * nn);
+    unsigned *flags = reinterpret_cast<unsigned *>(Cq + nc + nown);
     int *err = reinterpret_cast<int *>(flags + nflags);
     if (lds > 64 * 1024)
         RS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_sytrd_coop),
@@ -375,7 +699,8 @@ int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *
     void *args[] = {&H, &n, &cpw, &R, &D, &E, &TAU, &pub, &flags, &err};
     RS_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_sytrd_coop), dim3(nwg),
                                       dim3(kTrdThreads), args, (unsigned)lds, c.stream));
-    RS_BLAS(rocsolver_dstedc(c.blas, rocblas_evect_tridiagonal, n, D, E, V, n, einfo));
+    if (own) RS_TRY(tri_eig(n, D, E, V, einfo, Wt, Xt, Gt, Ft, FPt));
+    else RS_BLAS(rocsolver_dstedc(c.blas, rocblas_evect_tridiagonal, n, D, E, V, n, einfo));
     const dim3 qgrid((n + kQWaves - 1) / kQWaves), qblk(64 * kQWaves);
     const size_t qlds = sizeof(double) * 2 * kQBlk * 64;   // times NPL
     if (n <= 512) {

@@ -149,8 +149,9 @@ class RsdftGpu:
         self._call("rsdft_gpu_set_precond_mode", int(mode))
 
     def set_eigensolver(self, mode: int):
-        """1: refinement when Hsub is nearly diagonal, else mode 2 (default);
-        2: LDS-resident tridiagonal reduction + dstedc + dormtr; 0: always dsyevd."""
+        """1: refinement when Hsub is nearly diagonal, else mode 3 (default);
+        3: LDS-resident tridiagonal reduction + bisection / inverse iteration;
+        2: the reduction + dstedc; 0: always dsyevd."""
         self._call("rsdft_gpu_set_eigensolver", int(mode))
 
     def set_sweep_mode(self, mode: int):

@@ -1,11 +1,13 @@
 """The subspace eigensolvers (rsdft_gpu_test_eigh) against LAPACK's dsyevd
 (numpy.linalg.eigh) on symmetric matrices shaped like Hsub.
 
-Mode 2 is the LDS-resident Householder tridiagonalisation (one cooperative
-launch) + rocsolver_dstedc + the wave-per-columns back-transformation
-(rocsolver_dormtr above n = 512); mode 0 is rocsolver_dsyevd.  Bounds, with
-|H| the largest |eigenvalue|: eigenvalues 1e-12 |H|, residual |H V - V W|
-1e-11 |H|, orthogonality |V^T V - I| 1e-12.
+Modes 2 and 3 are the LDS-resident Householder tridiagonalisation (one
+cooperative launch) + the wave-per-columns back-transformation
+(rocsolver_dormtr above n = 512), with the tridiagonal eigenpairs from
+rocsolver_dstedc (2) or from bisection + inverse iteration (3; dstedc above
+n = 512); mode 0 is rocsolver_dsyevd.  Bounds, with |H| the largest
+|eigenvalue|: eigenvalues 1e-12 |H|, residual |H V - V W| 1e-11 |H|,
+orthogonality |V^T V - I| 1e-12.
 """
 import numpy as np
 import pytest
@@ -47,16 +49,17 @@ def check(H, w, V):
     assert np.all(np.diag(V) >= 0.0)
 
 
+@pytest.mark.parametrize("mode", [2, 3])
 @pytest.mark.parametrize("n", [3, 5, 33, 64, 65, 129, 257, 480, 600])
-def test_tridiagonal_eigensolver_shapes(dev, n):
-    print(f"case: test_eigh mode 2 hsub_like n={n}", flush=True)
+def test_tridiagonal_eigensolver_shapes(dev, n, mode):
+    print(f"case: test_eigh mode {mode} hsub_like n={n}", flush=True)
     H = hsub_like(n, seed=n)
-    w, V, ms = dev.test_eigh(H, mode=2)
+    w, V, ms = dev.test_eigh(H, mode=mode)
     print(f"  {ms:.3f} ms", flush=True)
     check(H, w, V)
 
 
-@pytest.mark.parametrize("mode", [2, 0])
+@pytest.mark.parametrize("mode", [3, 2, 0])
 def test_eigensolver_clustered(dev, mode):
     print(f"case: test_eigh mode {mode} clustered n=480", flush=True)
     H = clustered(480, seed=7)
@@ -65,15 +68,27 @@ def test_eigensolver_clustered(dev, mode):
     check(H, w, V)
 
 
-def test_tridiagonal_matches_dsyevd(dev):
+@pytest.mark.parametrize("mode", [2, 3])
+def test_tridiagonal_matches_dsyevd(dev, mode):
     """Same Ritz pairs as rocsolver_dsyevd (vectors up to the shared sign
     convention), at the Si216 subspace size."""
-    print("case: test_eigh mode 2 vs mode 0 n=480", flush=True)
+    print(f"case: test_eigh mode {mode} vs mode 0 n=480", flush=True)
     H = hsub_like(480, seed=11, mix=3e-2)
-    w2, V2, t2 = dev.test_eigh(H, mode=2)
+    w2, V2, t2 = dev.test_eigh(H, mode=mode)
     w0, V0, t0 = dev.test_eigh(H, mode=0)
-    print(f"  mode 2 {t2:.3f} ms, dsyevd {t0:.3f} ms", flush=True)
+    print(f"  mode {mode} {t2:.3f} ms, dsyevd {t0:.3f} ms", flush=True)
     assert np.max(np.abs(w2 - w0)) < 1e-12
     gap = np.min(np.diff(w0))
     assert gap > 1e-6
     assert np.max(np.abs(V2 - V0)) < 1e-8
+
+
+def test_tridiagonal_solver_exact_degeneracy(dev):
+    """Exactly repeated eigenvalues (20 copies of one 3x3 block, uncoupled):
+    bisection + inverse iteration must return an orthonormal basis of each
+    degenerate eigenspace."""
+    print("case: test_eigh mode 3 degenerate blocks n=60", flush=True)
+    blk = np.array([[1.0, 0.5, 0.0], [0.5, 2.0, 0.5], [0.0, 0.5, 3.0]])
+    H = np.kron(np.eye(20), blk)
+    w, V, ms = dev.test_eigh(H, mode=3)
+    check(H, w, V)

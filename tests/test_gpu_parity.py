@@ -60,11 +60,12 @@ def test_gram_schmidt_fp64(dev, oracle_si8, si8):
     assert rel(q, qo) < 1e-11
 
 
-@pytest.mark.parametrize("eig_mode", [1, 0, 2])
+@pytest.mark.parametrize("eig_mode", [1, 0, 2, 3])
 def test_subspace_diag_fp64(dev, oracle_si8, si8, eig_mode):
-    """eig_mode 1 (refinement, here falling back to the tridiagonal reduction:
-    Hsub of random vectors is far from diagonal), 0 rocsolver_dsyevd, 2 the
-    LDS-resident tridiagonal reduction + dstedc + dormtr."""
+    """eig_mode 1 (refinement, here falling back to mode 3: Hsub of random
+    vectors is far from diagonal), 0 rocsolver_dsyevd, 2 the LDS-resident
+    tridiagonal reduction + dstedc, 3 the reduction + bisection / inverse
+    iteration."""
     print(f"case: subspace_diag fp64 si8 eigensolver mode {eig_mode}", flush=True)
     q = oracle_si8.gram_schmidt(si8.psi0())
     dev.set_eigensolver(eig_mode)
