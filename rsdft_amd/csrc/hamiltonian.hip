@@ -213,6 +213,11 @@ k_nl_fill(float *__restrict__ out, long ML, int nb, int nrange, const int *__res
     if (w >= total) return;
     const int r = (int)(w % nrange);
     const int b0 = (int)(w / nrange) * NBK;
+    if (band_active) {   // the same decision in every thread: no band of the block active
+        bool any = false;
+        for (int k = 0; k < NBK && b0 + k < nb; ++k) any = any || band_active[b0 + k] != 0;
+        if (!any) return;
+    }
     const int g0 = rptr[r], ngr = rptr[r + 1] - g0;
     for (int t = threadIdx.x; t < ngr * P * NBK; t += blockDim.x) {
         const int slot = t / (P * NBK), rem = t - slot * (P * NBK);
