@@ -48,9 +48,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hpsi_traffic_si216.json")
 
 def hpsi_stencil_bytes(sysm, nb, s, acc):
     """Algorithmic HBM bytes of one all-band launch of the stencil kernel:
-    psi read + H psi write per band and Vloc once; accumulating (FP32 H psi)
-    also the read of the nonlocal term that the grid-order fill wrote into
-    H psi, one more band array."""
+    psi read + H psi write per band and Vloc once; accumulating (H psi) also
+    the read of the nonlocal term that the grid-order fill wrote into H psi,
+    one more band array."""
     ML = sysm.grid.ML
     b = 2 * ML * nb * s + ML * s
     if acc:
@@ -58,10 +58,10 @@ def hpsi_stencil_bytes(sysm, nb, s, acc):
     return b
 
 
-def nl_fill_bytes(sysm, nb):
-    """Algorithmic HBM bytes of one all-band launch of the FP32 grid-order
+def nl_fill_bytes(sysm, nb, s=4):
+    """Algorithmic HBM bytes of one all-band launch of the grid-order
     nonlocal fill: the term written at every grid point of every band."""
-    return sysm.grid.ML * nb * 4
+    return sysm.grid.ML * nb * s
 
 
 def pmc_summary(fetch_dir, write_dir, out_path):
@@ -270,7 +270,7 @@ def main():
     t_kernel_ms = float(hpm[4])
     B = hpsi_stencil_bytes(sysm, nb, 4, True)
     achieved = B / (t_kernel_ms * 1e-3) / 1e9
-    B64 = hpsi_stencil_bytes(sysm, nb, 8, False)
+    B64 = hpsi_stencil_bytes(sysm, nb, 8, True)
     ach64 = B64 / (float(hp64[4]) * 1e-3) / 1e9
     copy_gbs = 2 * sysm.grid.ML * nb * 4 / (float(hpm[7]) * 1e-3) / 1e9
     fill_gbs = nl_fill_bytes(sysm, nb) / (float(hpm[6]) * 1e-3) / 1e9

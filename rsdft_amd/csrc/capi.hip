@@ -412,17 +412,18 @@ int rsdft_gpu_set_projectors(int nzlma, int maxmjj, const int *MJJ, const int *J
         while (fill_mm < mmax) fill_mm *= 2;
         if (fill_mm * gp > 16) fill_mm = 0;   // per-thread uV registers
     }
-    std::vector<float> fuv;
+    std::vector<double> fuv64;
     if (fill_mm) {
-        fuv.assign((size_t)fill_mm * ML * gp, 0.0f);
+        fuv64.assign((size_t)fill_mm * ML * gp, 0.0);
         std::vector<uint8_t> seen(ML, 0);
         for (int g = 0; g < ng; ++g)
             for (int k = gptr[g]; k < gptr[g + 1]; ++k) {
                 const int i = gidx[k], m = seen[i]++;
                 for (int s2 = 0; s2 < gp; ++s2)
-                    fuv[((size_t)m * ML + i) * gp + s2] = (float)gval[(size_t)k * gp + s2];
+                    fuv64[((size_t)m * ML + i) * gp + s2] = gval[(size_t)k * gp + s2];
             }
     }
+    std::vector<float> fuv(fuv64.begin(), fuv64.end());
     std::vector<float> gval32(gval.begin(), gval.end());
     std::vector<double> scale(nzlma);
     for (int l = 0; l < nzlma; ++l) scale[l] = (double)iuV[l] * c.g.dV;
@@ -449,11 +450,14 @@ int rsdft_gpu_set_projectors(int nzlma, int maxmjj, const int *MJJ, const int *J
         RS_TRY(nl.fill_rgrp.ensure(sizeof(int) * std::max<size_t>(rgrp.size(), 1)));
         RS_TRY(nl.fill_meta.ensure(sizeof(uint32_t) * ML));
         RS_TRY(nl.fill_uv.ensure(sizeof(float) * fuv.size()));
+        RS_TRY(nl.fill_uv64.ensure(sizeof(double) * fuv64.size()));
         RS_HIP(hipMemcpy(nl.fill_rptr.p, rptr.data(), sizeof(int) * (nrange + 1), hipMemcpyHostToDevice));
         if (!rgrp.empty())
             RS_HIP(hipMemcpy(nl.fill_rgrp.p, rgrp.data(), sizeof(int) * rgrp.size(), hipMemcpyHostToDevice));
         RS_HIP(hipMemcpy(nl.fill_meta.p, meta.data(), sizeof(uint32_t) * ML, hipMemcpyHostToDevice));
         RS_HIP(hipMemcpy(nl.fill_uv.p, fuv.data(), sizeof(float) * fuv.size(), hipMemcpyHostToDevice));
+        RS_HIP(hipMemcpy(nl.fill_uv64.p, fuv64.data(), sizeof(double) * fuv64.size(),
+                         hipMemcpyHostToDevice));
     }
     RS_HIP(hipMemcpy(nl.scale64.p, scale.data(), sizeof(double) * nzlma, hipMemcpyHostToDevice));
     nl.nz = nzlma;
@@ -901,7 +905,7 @@ int rsdft_gpu_finalize(void)
                      &c.mat3, &c.mat4, &c.eval, &c.einfo, &c.evwork, &c.matT, &c.dots, &c.gemm_part, &c.opbuf, &c.pc_coef,
                      &c.cg.buf, &c.cg.flags, &c.nl.g_lma, &c.nl.g_ptr, &c.nl.g_idx, &c.nl.g_val64,
                      &c.nl.g_val32, &c.nl.g_order, &c.nl.scale64, &c.nl.d_ptr, &c.nl.d_idx, &c.nl.d_val,
-                     &c.nl.fill_rptr, &c.nl.fill_rgrp, &c.nl.fill_meta, &c.nl.fill_uv,
+                     &c.nl.fill_rptr, &c.nl.fill_rgrp, &c.nl.fill_meta, &c.nl.fill_uv, &c.nl.fill_uv64,
                      &c.nl.amap_dev};
     for (DevMem *m : all) m->release();
     if (c.blas) rocblas_destroy_handle(c.blas);

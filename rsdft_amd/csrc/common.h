@@ -16,7 +16,7 @@ constexpr int kMaxMd = 8;
 // thread, FP64 two x points tpr apart (strided); workgroup size limits
 constexpr int kHpsiPx32 = 2, kHpsiPx64 = 2;
 constexpr int kHpsiThreads32 = 1024, kHpsiThreads64 = 512;
-// FP32 H psi: the nonlocal term is written for every grid point of each band
+// H psi: the nonlocal term is written for every grid point of each band
 // (k_nl_fill, ranges of kFillRange points, at most kFillMaxM projector groups
 // covering a point and kFillMaxG groups touching a range) and the stencil
 // accumulates onto it
@@ -112,7 +112,7 @@ struct NonlocalTables {
     // fill_uv[(m*ML + i)*gp + s].  fill_mm = 0: the layout does not fit
     // (more than kFillMaxM memberships, kFillMaxG groups per range, ...)
     int fill_mm = 0, fill_gmax = 0, nrange = 0;
-    DevMem fill_rptr, fill_rgrp, fill_meta, fill_uv;
+    DevMem fill_rptr, fill_rgrp, fill_meta, fill_uv, fill_uv64;   // uV in FP32 / FP64
     DevMem scale64;             // iuV(lma) * dV
     // force tables
     int natom = 0;

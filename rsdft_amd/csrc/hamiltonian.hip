@@ -16,14 +16,14 @@
 //     reference's bcset_3 halo exchange (bcset_3_module.f90:15-151,
 //     bc_module.f90:722-736) has no counterpart;
 //   * the epilogue adds Vloc*psi (op_localpot);
-//   * FP32 H psi: the nonlocal term is first written at every grid point by
+//   * H psi: the nonlocal term is first written at every grid point by
 //     k_nl_fill (grid order, coefficients of each range's projector groups
 //     staged in LDS) and the stencil accumulates onto it, so the output is
 //     streamed twice instead of read-modified-written in the partial cache
-//     lines under the spheres; FP64 (and layouts the fill does not hold): the
-//     term is added afterwards by k_nl_apply, which visits only the grid
-//     points inside the projector spheres, a group of projectors sharing a
-//     sphere at a time, for many bands per table load.
+//     lines under the spheres; op_nonlocal (and layouts the fill does not
+//     hold) adds the term with k_nl_apply, which visits only the grid points
+//     inside the projector spheres, a group of projectors sharing a sphere at
+//     a time, for many bands per table load.
 #include "common.h"
 
 #include <algorithm>
@@ -187,26 +187,29 @@ k_nl_apply(T *__restrict__ out, long ML, int nb, const int *__restrict__ order, 
 }
 
 // ---------------------------------------------------------------------------
-// Nonlocal term written for every grid point (FP32 H psi), in grid order:
+// Nonlocal term written for every grid point (H psi), in grid order:
 //   out[b][i] = sum_{groups g covering i} sum_s uV(i, g, s) coefC[lma(g, s)][b]
 // (0 outside every sphere); the stencil then accumulates kinetic + local onto
 // it, so each output element is written by two streaming passes instead of a
 // read-modify-write of the partial cache lines under the spheres.  One
 // workgroup per (range of kFillRange points, NBK bands): the coefficients of
 // the range's groups are staged in LDS ([slot][s][band], rows padded by 4
-// words so different slots start in different banks), every thread owns 4
-// consecutive points (16-byte stores per band) and reads 4 bands per LDS access.
+// elements so different slots start in different banks), every thread owns 4
+// consecutive points (one 4-element vector store per band) and reads 4 bands
+// per LDS access.
 // ---------------------------------------------------------------------------
-template <int P, int MM, int NBK>
+template <class T, int P, int MM, int NBK>
 __global__ void __launch_bounds__(256)
-k_nl_fill(float *__restrict__ out, long ML, int nb, int nrange, const int *__restrict__ rptr,
+k_nl_fill(T *__restrict__ out, long ML, int nb, int nrange, const int *__restrict__ rptr,
           const int *__restrict__ rgrp, const int *__restrict__ glma,
-          const float *__restrict__ coefC, const unsigned *__restrict__ meta,
-          const float *__restrict__ uv, const int *__restrict__ band_active)
+          const T *__restrict__ coefC, const unsigned *__restrict__ meta,
+          const T *__restrict__ uv, const int *__restrict__ band_active)
 {
+    using V4 = T __attribute__((ext_vector_type(4)));
     constexpr int PT = 4, LS = P * NBK + 4;
     static_assert(kFillRange == 256 * PT, "one point quad per thread");
-    extern __shared__ float Lc[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char fill_raw[];
+    T *Lc = reinterpret_cast<T *>(fill_raw);
     const int nbb = (nb + NBK - 1) / NBK;
     const long total = (long)nrange * nbb;
     const long w = xcd_work_index(blockIdx.x, total);
@@ -224,7 +227,7 @@ k_nl_fill(float *__restrict__ out, long ML, int nb, int nrange, const int *__res
         const int s = rem / NBK, k = rem - s * NBK;
         const int lma = glma[rgrp[g0 + slot] * P + s];
         const int b = b0 + k;
-        Lc[slot * LS + rem] = (lma >= 0 && b < nb) ? coefC[(long)lma * nb + b] : 0.0f;
+        Lc[slot * LS + rem] = (lma >= 0 && b < nb) ? coefC[(long)lma * nb + b] : T(0);
     }
     __syncthreads();
     const long i0 = (long)r * kFillRange + threadIdx.x * PT;
@@ -237,13 +240,13 @@ k_nl_fill(float *__restrict__ out, long ML, int nb, int nrange, const int *__res
         cnt[p] = (int)(mt[p] & 7u);
         cmax = max(cmax, cnt[p]);
     }
-    float u[MM][PT * P];
+    T u[MM][PT * P];
 #pragma unroll
     for (int m = 0; m < MM; ++m) {
-        const float4 *src = reinterpret_cast<const float4 *>(uv + ((long)m * ML + i0) * P);
+        const V4 *src = reinterpret_cast<const V4 *>(uv + ((long)m * ML + i0) * P);
 #pragma unroll
         for (int q = 0; q < PT * P / 4; ++q) {
-            const float4 v = m < cmax ? src[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const V4 v = m < cmax ? src[q] : V4{T(0), T(0), T(0), T(0)};
             u[m][4 * q] = v.x;
             u[m][4 * q + 1] = v.y;
             u[m][4 * q + 2] = v.z;
@@ -251,11 +254,11 @@ k_nl_fill(float *__restrict__ out, long ML, int nb, int nrange, const int *__res
         }
     }
     for (int k0 = 0; k0 < NBK; k0 += 4) {
-        float acc[PT][4];
+        T acc[PT][4];
 #pragma unroll
         for (int p = 0; p < PT; ++p)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[p][j] = 0.0f;
+            for (int j = 0; j < 4; ++j) acc[p][j] = T(0);
 #pragma unroll
         for (int p = 0; p < PT; ++p)
 #pragma unroll
@@ -264,8 +267,8 @@ k_nl_fill(float *__restrict__ out, long ML, int nb, int nrange, const int *__res
                 const int base = (int)((mt[p] >> (3 + 6 * m)) & 63u) * LS + k0;
 #pragma unroll
                 for (int s = 0; s < P; ++s) {
-                    const float4 cv = *reinterpret_cast<const float4 *>(&Lc[base + s * NBK]);
-                    const float uu = u[m][p * P + s];
+                    const V4 cv = *reinterpret_cast<const V4 *>(&Lc[base + s * NBK]);
+                    const T uu = u[m][p * P + s];
                     acc[p][0] += uu * cv.x;
                     acc[p][1] += uu * cv.y;
                     acc[p][2] += uu * cv.z;
@@ -276,8 +279,8 @@ k_nl_fill(float *__restrict__ out, long ML, int nb, int nrange, const int *__res
         for (int j = 0; j < 4; ++j) {
             const int b = b0 + k0 + j;
             if (b >= nb || (band_active && !band_active[b])) continue;
-            *reinterpret_cast<float4 *>(out + (long)b * ML + i0) =
-                make_float4(acc[0][j], acc[1][j], acc[2][j], acc[3][j]);
+            *reinterpret_cast<V4 *>(out + (long)b * ML + i0) =
+                V4{acc[0][j], acc[1][j], acc[2][j], acc[3][j]};
         }
     }
 }
@@ -527,12 +530,8 @@ static int launch_hpsi_md(const T *in, T *out, int nb, int flags, const int *ban
         if (h1) launch_hpsi_k<T, MD, 1, kKin>(in, out, nb, band_active, vloc, cf);
         else launch_hpsi_k<T, MD, 2, kKin>(in, out, nb, band_active, vloc, cf);
     } else if (flags == (kKin | kLoc | kAcc)) {
-        if constexpr (sizeof(T) == 4) {
-            if (h1) launch_hpsi_k<T, MD, 1, kKin | kLoc | kAcc>(in, out, nb, band_active, vloc, cf);
-            else launch_hpsi_k<T, MD, 2, kKin | kLoc | kAcc>(in, out, nb, band_active, vloc, cf);
-        } else {
-            return set_error(kErrArg, "accumulating stencil is FP32 only");
-        }
+        if (h1) launch_hpsi_k<T, MD, 1, kKin | kLoc | kAcc>(in, out, nb, band_active, vloc, cf);
+        else launch_hpsi_k<T, MD, 2, kKin | kLoc | kAcc>(in, out, nb, band_active, vloc, cf);
     } else {
         return set_error(kErrArg, "stencil flags %d not supported", flags);
     }
@@ -647,28 +646,31 @@ int nl_apply(T *out, int nb, const T *coef, const int *band_active)
     return kOK;
 }
 
-template <int P, int MM>
-static void launch_nl_fill(int nb, const float *coef, const int *band_active, float *out)
+template <class T, int P, int MM>
+static void launch_nl_fill(int nb, const T *coef, const int *band_active, T *out)
 {
     Context &c = ctx();
     const NonlocalTables &nl = c.nl;
-    constexpr int NBK = P <= 4 ? 32 : 128 / P;
+    // LDS: fill_gmax (<= 64) slots x (P NBK + 4) elements stays below 64 KB
+    constexpr int NBK = sizeof(T) == 4 ? (P <= 4 ? 32 : 128 / P) : (P <= 4 ? 16 : 64 / P);
     constexpr int LS = P * NBK + 4;
     const long nblk = (long)nl.nrange * ((nb + NBK - 1) / NBK);
-    const size_t lds = sizeof(float) * (size_t)std::max(nl.fill_gmax, 1) * LS;
-    hipLaunchKernelGGL((k_nl_fill<P, MM, NBK>), dim3((unsigned)((nblk + 7) / 8 * 8)), dim3(256),
-                       lds, c.stream, out, c.g.ML, nb, nl.nrange, nl.fill_rptr.as<int>(),
-                       nl.fill_rgrp.as<int>(), nl.g_lma.as<int>(), coef,
-                       nl.fill_meta.as<unsigned>(), nl.fill_uv.as<float>(), band_active);
+    const size_t lds = sizeof(T) * (size_t)std::max(nl.fill_gmax, 1) * LS;
+    const T *uv = sizeof(T) == 8 ? (const T *)nl.fill_uv64.p : (const T *)nl.fill_uv.p;
+    hipLaunchKernelGGL((k_nl_fill<T, P, MM, NBK>), dim3((unsigned)((nblk + 7) / 8 * 8)),
+                       dim3(256), lds, c.stream, out, c.g.ML, nb, nl.nrange,
+                       nl.fill_rptr.as<int>(), nl.fill_rgrp.as<int>(), nl.g_lma.as<int>(), coef,
+                       nl.fill_meta.as<unsigned>(), uv, band_active);
 }
 
 // out[b][i] <- nonlocal term at every grid point of the active bands
-static int nl_fill(int nb, const float *coef, const int *band_active, float *out)
+template <class T>
+static int nl_fill(int nb, const T *coef, const int *band_active, T *out)
 {
     const NonlocalTables &nl = ctx().nl;
     switch (nl.gp * 8 + nl.fill_mm) {
 #define RS_FILL_CASE(P, MM) \
-    case P * 8 + MM: launch_nl_fill<P, MM>(nb, coef, band_active, out); break;
+    case P * 8 + MM: launch_nl_fill<T, P, MM>(nb, coef, band_active, out); break;
     RS_FILL_CASE(1, 1) RS_FILL_CASE(1, 2) RS_FILL_CASE(1, 4)
     RS_FILL_CASE(2, 1) RS_FILL_CASE(2, 2) RS_FILL_CASE(2, 4)
     RS_FILL_CASE(4, 1) RS_FILL_CASE(4, 2) RS_FILL_CASE(4, 4)
@@ -693,15 +695,13 @@ int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
         RS_TRY(c.coefC.ensure((size_t)nb * c.nl.nz * sizeof(T)));
         RS_TRY(nl_project<T>(in, nb, c.coefC.as<T>(), band_active));
     }
-    if constexpr (sizeof(T) == 4) {
-        // FP32 H psi: the nonlocal term written for every grid point, then the
-        // stencil accumulates kinetic + local onto it
-        if (sflags == (kKin | kLoc) && (flags & kNl) && fill_ready()) {
-            RS_TRY(nl_fill(nb, c.coefC.as<float>(), band_active, out));
-            RS_TRY(launch_stencil<T>(in, out, nb, kKin | kLoc | kAcc, band_active));
-            c.hpsi_bands += nb;
-            return kOK;
-        }
+    // H psi: the nonlocal term written for every grid point, then the stencil
+    // accumulates kinetic + local onto it
+    if (sflags == (kKin | kLoc) && (flags & kNl) && fill_ready()) {
+        RS_TRY(nl_fill<T>(nb, c.coefC.as<T>(), band_active, out));
+        RS_TRY(launch_stencil<T>(in, out, nb, kKin | kLoc | kAcc, band_active));
+        c.hpsi_bands += nb;
+        return kOK;
     }
     if (sflags & (kKin | kLoc)) {
         RS_TRY(launch_stencil<T>(in, out, nb, sflags, band_active));
@@ -716,16 +716,14 @@ int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
 template <class T>
 int hpsi_stencil_only(const T *in, T *out, int nb)
 {
-    if constexpr (sizeof(T) == 4)
-        if (fill_ready()) return launch_stencil<T>(in, out, nb, kKin | kLoc | kAcc, nullptr);
+    if (fill_ready()) return launch_stencil<T>(in, out, nb, kKin | kLoc | kAcc, nullptr);
     return launch_stencil<T>(in, out, nb, kKin | kLoc, nullptr);
 }
 
 template <class T>
 int nl_stage(T *out, int nb, const T *coef)
 {
-    if constexpr (sizeof(T) == 4)
-        if (fill_ready()) return nl_fill(nb, coef, nullptr, out);
+    if (fill_ready()) return nl_fill<T>(nb, coef, nullptr, out);
     return nl_apply<T>(out, nb, coef, nullptr);
 }
 
