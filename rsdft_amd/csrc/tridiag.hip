@@ -351,6 +351,7 @@ __global__ void k_trd_flag(const int *err, const int *info, int *flag)
 constexpr int kTriMax = 512;
 constexpr int kBisS = 2;
 constexpr int kInvRounds = 2;
+constexpr int kInvVPW = 2;   // eigenvectors per workgroup: (2 x 41 + 16) B x n <= 49 KB of LDS at n = 512
 constexpr double kClusterTol = 1e-7;
 constexpr double kEps = 1.1102230246251565e-16;   // 2^-53
 constexpr double kSafeMin = 2.2250738585072014e-308;
@@ -459,34 +460,48 @@ __global__ void k_tri_start(double *__restrict__ X, double *__restrict__ S, int 
     if (i == 0) S[j] = 1.0;
 }
 
-// one inverse-iteration solve per eigenvector, lane j of the grid owning
-// eigenvector j: X(i, j) = X[i*n + j] (consecutive lanes touch consecutive
-// words).  Round 0 first factors T - w_j I (dgttrf; the multipliers,
-// reciprocal pivots, first and second superdiagonals and row-interchange
-// flags stored in the same layout).  Every round solves with
-// Xin(:, j) * S[j] into Xout and sets S[j] = 1 / |Xout(:, j)|.  The
-// recurrences carry their values in registers, branches are selects, and
-// input, temporary and output are distinct arrays, so the loads of a sweep
-// are issued ahead of the dependency chain.
+// one inverse-iteration solve per eigenvector: kInvVPW eigenvectors per
+// 64-lane workgroup, lane t < kInvVPW running the sequential recurrences for
+// eigenvector j = blockIdx.x * kInvVPW + t with its LU factors (dgttrf: the
+// multipliers, reciprocal pivots, first and second superdiagonals and
+// row-interchange flags) and right-hand side in LDS ([i][t] layout), so the
+// recurrences carry their values in registers and wait only on LDS loads
+// that do not depend on them.  The whole wave moves the columns between
+// global memory (X(i, j) = X[i*n + j]) and LDS.  Each round refactors
+// T - w_j I (LDS does not outlive the launch) and solves with
+// Xin(:, j) * S[j] into Xout, S[j] = 1 / |Xout(:, j)|.
 __global__ void __launch_bounds__(64)
 k_tri_invit(const double *__restrict__ d, const double *__restrict__ e,
-            const double *__restrict__ w, const double *__restrict__ tnorm_p, int n, int round,
-            double *__restrict__ FL, double *__restrict__ FR, double *__restrict__ FU,
-            double *__restrict__ FU2, unsigned char *__restrict__ FP,
-            const double *__restrict__ Xin, double *__restrict__ Y, double *__restrict__ Xout,
-            double *__restrict__ S)
+            const double *__restrict__ w, const double *__restrict__ tnorm_p, int n,
+            const double *__restrict__ Xin, double *__restrict__ Xout, double *__restrict__ S)
 {
-    const int j = blockIdx.x * 64 + threadIdx.x;
-    if (j >= n) return;
+    extern __shared__ __attribute__((aligned(16))) double tsm[];
+    constexpr int V = kInvVPW;
+    const int t = threadIdx.x, j0 = blockIdx.x * V;
+    const int nv = min(V, n - j0);
     const long N = n;
-    if (round == 0) {
+    double *FL = tsm, *FR = FL + V * n, *FU = FR + V * n, *FU2 = FU + V * n, *B = FU2 + V * n;
+    double *sd = B + V * n, *se = sd + n;
+    unsigned char *FP = reinterpret_cast<unsigned char *>(se + n);
+    // T and the right-hand sides Xin(:, j) * S[j] into LDS, all lanes
+    for (int i = t; i < n; i += 64) {
+        sd[i] = d[i];
+        se[i] = i < n - 1 ? e[i] : 0.0;
+    }
+    for (int q = t; q < nv * n; q += 64) {
+        const int v = q / n, i = q - v * n;
+        B[i * V + v] = Xin[(long)i * N + j0 + v] * S[j0 + v];
+    }
+    __syncthreads();
+    if (t < nv) {
+        const int j = j0 + t;
         const double lam = w[j];
         const double ptol = fmax(kEps * *tnorm_p, kSafeMin);
-        double dc = d[0] - lam, duc = n > 1 ? e[0] : 0.0;   // current D(i), DU(i)
-#pragma unroll 4
+        // dgttrf
+        double dc = sd[0] - lam, duc = se[0];   // current D(i), DU(i)
+#pragma unroll 8
         for (int i = 0; i < n - 1; ++i) {
-            const double li = e[i], dn = d[i + 1] - lam;
-            const double dun = i < n - 2 ? e[i + 1] : 0.0;
+            const double li = se[i], dn = sd[i + 1] - lam, dun = se[i + 1];   // se[n-1] = 0
             const bool keep = fabs(dc) >= fabs(li);   // no row interchange
             const double num = keep ? li : dc;
             double den = keep ? dc : li;
@@ -496,68 +511,68 @@ k_tri_invit(const double *__restrict__ d, const double *__restrict__ e,
             const double ndc = keep ? dn - f * duc : duc - f * dn;
             const double nduc = keep ? dun : -f * dun;
             if (fabs(piv) < ptol) piv = piv < 0.0 ? -ptol : ptol;
-            const long ix = (long)i * N + j;
-            FL[ix] = f;
-            FR[ix] = tri_recip(piv);
-            FU[ix] = keep ? duc : dn;
-            FU2[ix] = keep ? 0.0 : dun;
-            FP[ix] = keep ? 0 : 1;
+            FL[i * V + t] = f;
+            FR[i * V + t] = tri_recip(piv);
+            FU[i * V + t] = keep ? duc : dn;
+            FU2[i * V + t] = keep ? 0.0 : dun;
+            FP[i * V + t] = keep ? 0 : 1;
             dc = ndc;
             duc = nduc;
         }
         if (fabs(dc) < ptol) dc = dc < 0.0 ? -ptol : ptol;
-        FR[(long)(n - 1) * N + j] = tri_recip(dc);
-    }
-    // P and L: Y = L^-1 P b
-    const double sin = S[j];
-    double bc = Xin[j] * sin;
+        FR[(n - 1) * V + t] = tri_recip(dc);
+        // P and L in place: step i writes B(i) and reads B(i+1), not yet written
+        double bc = B[t];
 #pragma unroll 8
-    for (int i = 0; i < n - 1; ++i) {
-        const long ix = (long)i * N + j;
-        const double bn = Xin[ix + N] * sin;
-        const double f = FL[ix];
-        const bool sw = FP[ix] != 0;
-        const double y = sw ? bn : bc, o = sw ? bc : bn;
-        Y[ix] = y;
-        bc = o - f * y;
-    }
-    Y[(long)(n - 1) * N + j] = bc;
-    // U: Xout = U^-1 Y with the sum of squares.  A block of 8 steps grows |x|
-    // by at most (2 |T| / ptol)^8 < 1e132, so rescaling by 1e-100 after every
-    // block in which |x| passed 1e100 keeps |x| below 1e232.  The block's
-    // squares are summed as (x 1e-100)^2 (<= 1e264, finite): that is the
-    // square of the rescaled value, or 1e-200 x^2 without rescale.
-    double x1 = 0.0, x2 = 0.0, ysc = 1.0, ss = 0.0;   // x(i+1), x(i+2), scale of the rest of Y
-    int i = n - 1;
-    while (i >= 0) {
-        const int i1 = max(i - 7, 0);
-        double mx = 0.0, sb = 0.0;
+        for (int i = 0; i < n - 1; ++i) {
+            const double bn = B[(i + 1) * V + t], f = FL[i * V + t];
+            const bool sw = FP[i * V + t] != 0;
+            const double y = sw ? bn : bc, o = sw ? bc : bn;
+            B[i * V + t] = y;
+            bc = o - f * y;
+        }
+        B[(n - 1) * V + t] = bc;
+        // U in place with the sum of squares.  A block of 8 steps grows |x|
+        // by at most (2 |T| / ptol)^8 < 1e132, so rescaling by 1e-100 after
+        // every block in which |x| passed 1e100 keeps |x| below 1e232.  The
+        // block's squares are summed as (x 1e-100)^2 (<= 1e264, finite): the
+        // square of the rescaled value, or 1e-200 x^2 without rescale.
+        double x1 = 0.0, x2 = 0.0, ysc = 1.0, ss = 0.0;   // x(i+1), x(i+2), scale of the rest of Y
+        int i = n - 1;
+        while (i >= 0) {
+            const int i1 = max(i - 7, 0);
+            double mx = 0.0, sb = 0.0;
 #pragma unroll 8
-        for (int k = 0; k < 8; ++k) {
-            const int ii = i - k;
-            if (ii < i1) break;
-            const long ix = (long)ii * N + j;
-            const double du = ii < n - 1 ? FU[ix] : 0.0;
-            const double du2 = ii < n - 2 ? FU2[ix] : 0.0;
-            const double x = (Y[ix] * ysc - du * x1 - du2 * x2) * FR[ix];
-            Xout[ix] = x;
-            mx = fmax(mx, fabs(x));
-            sb += (x * 1e-100) * (x * 1e-100);
-            x2 = x1;
-            x1 = x;
+            for (int k = 0; k < 8; ++k) {
+                const int ii = i - k;
+                if (ii < i1) break;
+                const double du = ii < n - 1 ? FU[ii * V + t] : 0.0;
+                const double du2 = ii < n - 2 ? FU2[ii * V + t] : 0.0;
+                const double x = (B[ii * V + t] * ysc - du * x1 - du2 * x2) * FR[ii * V + t];
+                B[ii * V + t] = x;
+                mx = fmax(mx, fabs(x));
+                sb += (x * 1e-100) * (x * 1e-100);
+                x2 = x1;
+                x1 = x;
+            }
+            if (mx > 1e100) {   // rescale the computed tail (rare)
+                for (int k = i1; k < n; ++k) B[k * V + t] *= 1e-100;
+                x1 *= 1e-100;
+                x2 *= 1e-100;
+                ysc *= 1e-100;
+                ss = ss * 1e-200 + sb;
+            } else {
+                ss += sb * 1e200;   // sb <= 8 here
+            }
+            i = i1 - 1;
         }
-        if (mx > 1e100) {   // rescale the computed tail (rare)
-            for (int k = i1; k < n; ++k) Xout[(long)k * N + j] *= 1e-100;
-            x1 *= 1e-100;
-            x2 *= 1e-100;
-            ysc *= 1e-100;
-            ss = ss * 1e-200 + sb;
-        } else {
-            ss += sb * 1e200;   // sb <= 8 here
-        }
-        i = i1 - 1;
+        S[j] = 1.0 / sqrt(ss);
     }
-    S[j] = 1.0 / sqrt(ss);
+    __syncthreads();
+    for (int q = t; q < nv * n; q += 64) {
+        const int v = q / n, i = q - v * n;
+        Xout[(long)i * N + j0 + v] = B[i * V + v];
+    }
 }
 
 // Xout(:, j) *= S[j] for the members of clusters (the others keep the factor),
@@ -609,22 +624,20 @@ __global__ void k_tri_identity(double *G, int n)
 }
 
 // eigenvalues of T into D (ascending, replacing the diagonal) and
-// eigenvectors into V; scratch: W (2n + 8), X (3 n x n: two iterates and the
-// forward-solve temporary), G and the four factor arrays (n x n each), FP
-// (n x n bytes)
+// eigenvectors into V; scratch: W (2n + 8), X (2 n x n: two iterates), G (n x n)
 static int tri_eig(int n, double *D, const double *E, double *V, int *einfo, double *W,
-                   double *X, double *G, double *F, unsigned char *FP)
+                   double *X, double *G)
 {
     Context &c = ctx();
     const size_t nn = (size_t)n * n;
     double *tn = W + n, *S = W + n + 8;
     hipLaunchKernelGGL(k_tri_bisect, dim3(n), dim3(64), 0, c.stream, D, E, n, W, tn, einfo);
-    const dim3 ig((n + 63) / 64);
-    double *Xa = X, *Xb = X + nn, *Y = X + 2 * nn;
+    const dim3 ig((n + kInvVPW - 1) / kInvVPW);
+    const size_t lds = ((5 * sizeof(double) + 1) * kInvVPW + 2 * sizeof(double)) * (size_t)n;
+    double *Xa = X, *Xb = X + nn;
     hipLaunchKernelGGL(k_tri_start, dim3((n + 127) / 128, n), dim3(128), 0, c.stream, Xa, S, n);
     for (int r = 0; r < kInvRounds; ++r) {
-        hipLaunchKernelGGL(k_tri_invit, ig, dim3(64), 0, c.stream, D, E, W, tn, n, r, F, F + nn,
-                           F + 2 * nn, F + 3 * nn, FP, Xa, Y, Xb, S);
+        hipLaunchKernelGGL(k_tri_invit, ig, dim3(64), lds, c.stream, D, E, W, tn, n, Xa, Xb, S);
         hipLaunchKernelGGL(k_tri_mgs, dim3(n), dim3(64), 0, c.stream, W, tn, n, Xb, S);
         std::swap(Xa, Xb);
     }
@@ -681,14 +694,13 @@ int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *
     const int nqb = (n - 1 + kQBlk - 1) / kQBlk;
     const size_t nc = (size_t)nqb * kQBlk * kQBlk;
     const bool own = tri_solver == 1 && n <= kTriMax;
-    // tri_eig scratch: W, 3 X, G, 4 factor arrays, the interchange flags (bytes)
-    const size_t nown = own ? 8 * nn + 2 * (size_t)n + 8 + (nn + 7) / 8 : 0;
+    // tri_eig scratch: W, 2 X, G
+    const size_t nown = own ? 3 * nn + 2 * (size_t)n + 8 : 0;
     RS_TRY(c.evwork.ensure(sizeof(double) * (nn + 6 * (size_t)n + nc + nown) +
                            sizeof(int) * (nflags + 4)));
     double *R = c.evwork.as<double>(), *E = R + nn, *TAU = E + n, *pub = TAU + n;
     double *Cq = pub + 4 * (size_t)n;
-    double *Wt = Cq + nc, *Xt = Wt + 2 * n + 8, *Gt = Xt + 3 * nn, *Ft = Gt + nn;
-    unsigned char *FPt = reinterpret_cast<unsigned char *>(Ft + 4 * nn);
+    double *Wt = Cq + nc, *Xt = Wt + 2 * n + 8, *Gt = Xt + 2 * nn;
     unsigned *flags = reinterpret_cast<unsigned *>(Cq + nc + nown);
     int *err = reinterpret_cast<int *>(flags + nflags);
     if (lds > 64 * 1024)
@@ -699,7 +711,7 @@ int eigh_tridiag(const double *H, int n, double *V, double *D, int *einfo, int *
     void *args[] = {&H, &n, &cpw, &R, &D, &E, &TAU, &pub, &flags, &err};
     RS_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_sytrd_coop), dim3(nwg),
                                       dim3(kTrdThreads), args, (unsigned)lds, c.stream));
-    if (own) RS_TRY(tri_eig(n, D, E, V, einfo, Wt, Xt, Gt, Ft, FPt));
+    if (own) RS_TRY(tri_eig(n, D, E, V, einfo, Wt, Xt, Gt));
     else RS_BLAS(rocsolver_dstedc(c.blas, rocblas_evect_tridiagonal, n, D, E, V, n, einfo));
     const dim3 qgrid((n + kQWaves - 1) / kQWaves), qblk(64 * kQWaves);
     const size_t qlds = sizeof(double) * 2 * kQBlk * 64;   // times NPL
