@@ -151,11 +151,12 @@ RSDFT_API int rsdft_gpu_band_energy_grad(const double *unk, int ML0, int MB, con
 /* test_hpsi2 (test_hpsi2_module.f90:16-143): times nrep applications to the
  * attached wavefunctions on the library's stream; times_ms[8] = mean ms per
  * all-band application of [0] hamiltonian, [1] op_kinetic, [2] op_localpot,
- * [3] op_nonlocal, [4] the stencil kernel as the hamiltonian runs it (FP32:
- * kinetic + local accumulated onto the nonlocal term; FP64: kinetic + local),
- * [5] the projection kernel, [6] the nonlocal term stage (FP32: written at
- * every grid point in grid order; FP64: scatter-added at the sphere points),
- * [7] a plain copy of all bands (the bandwidth reference). */
+ * [3] op_nonlocal, [4] the stencil kernel as the hamiltonian runs it
+ * (kinetic + local accumulated onto the nonlocal term), [5] the projection
+ * kernel, [6] the nonlocal term stage (written at every grid point in grid
+ * order; scatter-added at the sphere points when a point lies in more than 4
+ * projector spheres), [7] a plain copy of all bands (the bandwidth
+ * reference). */
 RSDFT_API int rsdft_gpu_test_hpsi(int nrep, double *times_ms);
 
 /* Diagnostics of the subspace eigensolve (the dsyevd call of

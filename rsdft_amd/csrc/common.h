@@ -104,7 +104,7 @@ struct NonlocalTables {
     DevMem g_val64, g_val32;    // [npts*gp] uV of each slot at each point (0 outside its sphere)
     DevMem g_order;             // groups ordered by colour
     std::vector<int> color_ptr; // colour k: g_order[color_ptr[k] .. color_ptr[k+1])
-    // grid-order fill of the nonlocal term (FP32 H psi): the grid cut into
+    // grid-order fill of the nonlocal term (H psi): the grid cut into
     // ranges of kFillRange points, the groups touching each range (CSR
     // fill_rptr/fill_rgrp), and per grid point its membership count and the
     // range-local slot of each membership (fill_meta: bits 0-2 count, 6 bits
@@ -204,10 +204,10 @@ int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active = 
 template <class T>
 int nl_project(const T *in, int nb, T *coef, const int *band_active = nullptr);
 // The pieces of one H psi, for timing: the stencil kernel as hpsi_apply runs
-// it (FP32 with the fill layout: kinetic + local accumulated onto out; else
-// kinetic + local), the nonlocal term stage (FP32 with the fill layout: the
-// term written for every grid point; else scatter-added into out), and a
-// plain copy of nb bands.
+// it (with the fill layout: kinetic + local accumulated onto out; else
+// kinetic + local), the nonlocal term stage (with the fill layout: the term
+// written for every grid point; else scatter-added into out), and a plain
+// copy of nb bands.
 template <class T>
 int hpsi_stencil_only(const T *in, T *out, int nb);
 template <class T>
