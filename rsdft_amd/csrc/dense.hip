@@ -135,10 +135,10 @@ __global__ void k_rs_norms(const double *__restrict__ S, const double *__restric
 {
     __shared__ double red[3][16];
     double a = 0.0, b = 0.0, h = 0.0;
-    const long nn = (long)n * n;
-    for (long t = threadIdx.x; t < nn; t += blockDim.x) {
-        const int j = (int)(t / n), i = (int)(t - (long)j * n);
-        if (i != j) a = fmax(a, fabs(S[t]));
+    const int nn = n * n;   // n <= 46340
+#pragma unroll 4
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+        if (t % (n + 1) != 0) a = fmax(a, fabs(S[t]));   // off the diagonal
         if (R) b = fmax(b, fabs(R[t]));
         if (hnorm) h = fmax(h, fabs(H[t]));
     }
@@ -190,9 +190,10 @@ k_rs_clusters(const double *__restrict__ B, const double *__restrict__ R, int n,
     }
     if (tid == 0) { npairs = 0; fail = 0; maxsz = 0; }
     __syncthreads();
-    const long nn = (long)n * n;
-    for (long t = tid; t < nn; t += nt) {
-        const int j = (int)(t / n), i = (int)(t - (long)j * n);
+    const int nn = n * n;   // n <= 46340
+#pragma unroll 4
+    for (int t = tid; t < nn; t += nt) {
+        const int j = t / n, i = t - j * n;
         if (i >= j) continue;
         const double num = B[t] + l[j] * (R ? R[t] : 0.0);
         const double gap = l[j] - l[i];
