@@ -467,19 +467,33 @@ __global__ void k_tri_start(double *__restrict__ X, double *__restrict__ S, int 
 // row-interchange flags) and right-hand side in LDS ([i][t] layout), so the
 // recurrences carry their values in registers and wait only on LDS loads
 // that do not depend on them.  The whole wave moves the columns between
-// global memory (X(i, j) = X[i*n + j]) and LDS.  Each round refactors
-// T - w_j I (LDS does not outlive the launch) and solves with
-// Xin(:, j) * S[j] into Xout, S[j] = 1 / |Xout(:, j)|.
+// global memory (X(i, j) = X[i*n + j]) and LDS.  A round factors
+// T - w_j I and solves with Xin(:, j) * S[j] into Xout, S[j] = 1 / |Xout(:, j)|.
+// A workgroup none of whose eigenvectors lies in a cluster (where Gram-Schmidt
+// must run between the rounds) does all kInvRounds solves in round 0 with the
+// same factors, writes Xfinal (the buffer the last round writes) and skips the
+// later rounds.
 __global__ void __launch_bounds__(64)
 k_tri_invit(const double *__restrict__ d, const double *__restrict__ e,
-            const double *__restrict__ w, const double *__restrict__ tnorm_p, int n,
-            const double *__restrict__ Xin, double *__restrict__ Xout, double *__restrict__ S)
+            const double *__restrict__ w, const double *__restrict__ tnorm_p, int n, int round,
+            const double *__restrict__ Xin, double *__restrict__ Xout,
+            double *__restrict__ Xfinal, double *__restrict__ S)
 {
     extern __shared__ __attribute__((aligned(16))) double tsm[];
     constexpr int V = kInvVPW;
     const int t = threadIdx.x, j0 = blockIdx.x * V;
     const int nv = min(V, n - j0);
     const long N = n;
+    const double ctol = kClusterTol * *tnorm_p;
+    bool alone = true;   // the same in every thread
+    for (int v = 0; v < nv; ++v) {
+        const int j = j0 + v;
+        if ((j > 0 && w[j] - w[j - 1] < ctol) || (j + 1 < n && w[j + 1] - w[j] < ctol))
+            alone = false;
+    }
+    if (alone && round > 0) return;   // finished in round 0
+    const int nsolve = alone ? kInvRounds : 1;
+    double *out = alone ? Xfinal : Xout;
     double *FL = tsm, *FR = FL + V * n, *FU = FR + V * n, *FU2 = FU + V * n, *B = FU2 + V * n;
     double *sd = B + V * n, *se = sd + n;
     unsigned char *FP = reinterpret_cast<unsigned char *>(se + n);
@@ -521,6 +535,11 @@ k_tri_invit(const double *__restrict__ d, const double *__restrict__ e,
         }
         if (fabs(dc) < ptol) dc = dc < 0.0 ? -ptol : ptol;
         FR[(n - 1) * V + t] = tri_recip(dc);
+        for (int sv = 0; sv < nsolve; ++sv) {
+        if (sv > 0) {   // the next right-hand side: the normalised solution
+            const double sc = S[j];
+            for (int i = 0; i < n; ++i) B[i * V + t] *= sc;
+        }
         // P and L in place: step i writes B(i) and reads B(i+1), not yet written
         double bc = B[t];
 #pragma unroll 8
@@ -567,11 +586,12 @@ k_tri_invit(const double *__restrict__ d, const double *__restrict__ e,
             i = i1 - 1;
         }
         S[j] = 1.0 / sqrt(ss);
+        }
     }
     __syncthreads();
     for (int q = t; q < nv * n; q += 64) {
         const int v = q / n, i = q - v * n;
-        Xout[(long)i * N + j0 + v] = B[i * V + v];
+        out[(long)i * N + j0 + v] = B[i * V + v];
     }
 }
 
@@ -635,9 +655,11 @@ static int tri_eig(int n, double *D, const double *E, double *V, int *einfo, dou
     const dim3 ig((n + kInvVPW - 1) / kInvVPW);
     const size_t lds = ((5 * sizeof(double) + 1) * kInvVPW + 2 * sizeof(double)) * (size_t)n;
     double *Xa = X, *Xb = X + nn;
+    double *Xfinal = kInvRounds % 2 == 0 ? Xa : Xb;   // where the last round writes
     hipLaunchKernelGGL(k_tri_start, dim3((n + 127) / 128, n), dim3(128), 0, c.stream, Xa, S, n);
     for (int r = 0; r < kInvRounds; ++r) {
-        hipLaunchKernelGGL(k_tri_invit, ig, dim3(64), lds, c.stream, D, E, W, tn, n, Xa, Xb, S);
+        hipLaunchKernelGGL(k_tri_invit, ig, dim3(64), lds, c.stream, D, E, W, tn, n, r, Xa, Xb,
+                           Xfinal, S);
         hipLaunchKernelGGL(k_tri_mgs, dim3(n), dim3(64), 0, c.stream, W, tn, n, Xb, S);
         std::swap(Xa, Xb);
     }

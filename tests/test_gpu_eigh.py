@@ -92,3 +92,20 @@ def test_tridiagonal_solver_exact_degeneracy(dev):
     H = np.kron(np.eye(20), blk)
     w, V, ms = dev.test_eigh(H, mode=3)
     check(H, w, V)
+
+
+def test_tridiagonal_solver_mixed_clusters(dev):
+    """Half the spectrum well separated, half in exact pairs: workgroups with
+    and without cluster members in one solve (the former finish all
+    inverse-iteration rounds in one launch, the latter orthogonalise between
+    rounds)."""
+    print("case: test_eigh mode 3 mixed separated / paired spectrum n=300", flush=True)
+    rng = np.random.default_rng(5)
+    single = rng.uniform(-1.0, 0.0, 150)
+    pairs = rng.uniform(0.0, 1.0, 75)
+    lam = np.sort(np.concatenate([single, pairs, pairs]))
+    q, _ = np.linalg.qr(rng.standard_normal((300, 300)))
+    H = (q * lam) @ q.T
+    H = 0.5 * (H + H.T)
+    w, V, ms = dev.test_eigh(H, mode=3)
+    check(H, w, V)
