@@ -1,11 +1,11 @@
 """Grids other than the cubic configurations, through the C-ABI against the
 oracle: an anisotropic 20 x 24 x 28 cell (different stencil tiling in every
 direction, ranges of the grid-order nonlocal fill that end inside planes)
-with three atoms, for the hamiltonian in both precisions and the FP64
-subspace step; and a grid the stencil kernels cannot tile (x extent not a
+with three atoms, for the hamiltonian in both precisions, the FP64 subspace
+step and two FP64 sweeps; and a grid the stencil kernels cannot tile (x extent not a
 multiple of 4), which must be refused at initialisation with an error.
 Bounds as in the Si8 tests: H psi 1e-12 (FP64) and 2e-6 (FP32 storage)
-relative, Ritz values 1e-10 Ha.
+relative, Ritz values 1e-10 Ha, spans orthogonal to 1e-9.
 """
 import numpy as np
 import pytest
@@ -50,6 +50,24 @@ def test_subspace_diag_anisotropic(gpu_ctx, aniso, aniso_oracle):
     xo, eo, _ = aniso_oracle.subspace_diag(q)
     print(f"  max |d eig| {np.max(np.abs(e - eo)):.2e} Ha", flush=True)
     assert np.max(np.abs(e - eo)) < 1e-10
+
+
+@pytest.mark.parametrize("pc_mode", [1, 0])
+def test_two_sweeps_anisotropic(gpu_ctx, aniso, aniso_oracle, pc_mode):
+    """CG (both preconditioner forms), Gram-Schmidt and subspace step on the
+    anisotropic cell: Ritz values and the orthonormal span after each sweep."""
+    print(f"case: 2 FP64 sweeps, 20x24x28 cell, precond mode {pc_mode}", flush=True)
+    dev = RsdftGpu(aniso, precision=PREC_FP64)
+    dev.set_precond_mode(pc_mode)
+    x = aniso_oracle.gram_schmidt(aniso.psi0())
+    xg = x.copy()
+    for it in range(2):
+        x, eo, _ = aniso_oracle.sweep(x)
+        xg, eg, _ = dev.sweep(xg, 1)
+        print(f"  sweep {it}: max|de| = {np.max(np.abs(eg - eo)):.3e}", flush=True)
+        assert np.max(np.abs(eg - eo)) < 1e-10
+    m = aniso.grid.dV * (xg @ x.T)
+    assert np.max(np.abs(m.T @ m - np.eye(m.shape[0]))) < 1e-9
 
 
 def test_untileable_grid_is_refused(gpu_ctx):
