@@ -2,10 +2,11 @@
 oracle: an anisotropic 20 x 24 x 28 cell (different stencil tiling in every
 direction, ranges of the grid-order nonlocal fill that end inside planes)
 with three atoms, for the hamiltonian in both precisions, the FP64 subspace
-step and two FP64 sweeps; and a grid the stencil kernels cannot tile (x extent not a
+step, two FP64 sweeps and the mixed-precision sweeps; and a grid the stencil kernels cannot tile (x extent not a
 multiple of 4), which must be refused at initialisation with an error.
 Bounds as in the Si8 tests: H psi 1e-12 (FP64) and 2e-6 (FP32 storage)
-relative, Ritz values 1e-10 Ha, spans orthogonal to 1e-9.
+relative, Ritz values 1e-10 Ha (1e-6 Ha after mixed sweeps), spans orthogonal
+to 1e-9.
 """
 import numpy as np
 import pytest
@@ -68,6 +69,22 @@ def test_two_sweeps_anisotropic(gpu_ctx, aniso, aniso_oracle, pc_mode):
         assert np.max(np.abs(eg - eo)) < 1e-10
     m = aniso.grid.dV * (xg @ x.T)
     assert np.max(np.abs(m.T @ m - np.eye(m.shape[0]))) < 1e-9
+
+
+def test_mixed_sweeps_anisotropic(gpu_ctx, aniso, aniso_oracle):
+    """4 FP32 sweeps + 2 FP64 sweeps against 6 FP64 oracle sweeps (< 1e-6 Ha)."""
+    print("case: 4 mixed + 2 FP64 sweeps, 20x24x28 cell", flush=True)
+    q = aniso_oracle.gram_schmidt(aniso.psi0())
+    x = q
+    for _ in range(6):
+        x, eo, _ = aniso_oracle.sweep(x)
+    dev = RsdftGpu(aniso, precision=PREC_MIXED_FP32)
+    xm, em, _ = dev.sweep(q, 4)
+    dev.set_precision(PREC_FP64)
+    xm, em, _ = dev.sweep(xm, 2)
+    err = float(np.max(np.abs(em - eo)))
+    print(f"  max |eig err| = {err:.3e} Ha", flush=True)
+    assert err < 1e-6, err
 
 
 def test_untileable_grid_is_refused(gpu_ctx):
