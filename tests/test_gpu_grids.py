@@ -2,7 +2,8 @@
 oracle: an anisotropic 20 x 24 x 28 cell (different stencil tiling in every
 direction, ranges of the grid-order nonlocal fill that end inside planes)
 with three atoms, for the hamiltonian in both precisions, the FP64 subspace
-step, two FP64 sweeps and the mixed-precision sweeps; and a grid the stencil kernels cannot tile (x extent not a
+step, two FP64 sweeps, the mixed-precision sweeps, the density and the
+nonlocal force; and a grid the stencil kernels cannot tile (x extent not a
 multiple of 4), which must be refused at initialisation with an error.
 Bounds as in the Si8 tests: H psi 1e-12 (FP64) and 2e-6 (FP32 storage)
 relative, Ritz values 1e-10 Ha (1e-6 Ha after mixed sweeps), spans orthogonal
@@ -85,6 +86,21 @@ def test_mixed_sweeps_anisotropic(gpu_ctx, aniso, aniso_oracle):
     err = float(np.max(np.abs(em - eo)))
     print(f"  max |eig err| = {err:.3e} Ha", flush=True)
     assert err < 1e-6, err
+
+
+def test_density_and_force_anisotropic(gpu_ctx, aniso, aniso_oracle):
+    """The adjoint outputs: density (1e-13) and nonlocal force (1e-11)."""
+    print("case: FP64 density and nonlocal force, 20x24x28 cell", flush=True)
+    dev = RsdftGpu(aniso, precision=PREC_FP64)
+    q = aniso_oracle.gram_schmidt(aniso.psi0())
+    rho = dev.calc_density(q, aniso.occ)
+    ro = aniso_oracle.density(q, aniso.occ)
+    f = dev.calc_force_ps_nloc2(q, aniso.occ)
+    fo = aniso_oracle.force_nloc(q, aniso.occ)
+    er = float(np.max(np.abs(rho - ro)) / np.max(np.abs(ro)))
+    ef = float(np.max(np.abs(f - fo)) / np.max(np.abs(fo)))
+    print(f"  density {er:.2e}, force {ef:.2e}", flush=True)
+    assert er < 1e-13 and ef < 1e-11, (er, ef)
 
 
 def test_untileable_grid_is_refused(gpu_ctx):
