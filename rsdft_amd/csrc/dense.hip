@@ -129,15 +129,18 @@ __global__ void k_set_identity(double *A, int n)
     if (i < n) A[(long)j * n + i] = i == j ? 1.0 : 0.0;
 }
 
-// st[1] = max |offdiag B|, st[2] = max |R|, st[3] = max |H| (when hnorm); R may be null
-__global__ void k_rs_norms(const double *__restrict__ S, const double *__restrict__ R,
-                           const double *__restrict__ H, int n, int hnorm, double *st)
+// st[1] = max |offdiag B|, st[2] = max |R|, st[3] = max |H| (when hnorm); R may
+// be null.  Grid-stride over the n^2 entries; every workgroup folds its maxima
+// into st by an atomic max on the bit patterns (non-negative doubles order like
+// them), so those entries must be zero on entry (k_rs_clear_norms).
+__global__ void __launch_bounds__(256)
+k_rs_norms(const double *__restrict__ S, const double *__restrict__ R,
+           const double *__restrict__ H, int n, int hnorm, double *st)
 {
-    __shared__ double red[3][16];
+    __shared__ double red[3][4];
     double a = 0.0, b = 0.0, h = 0.0;
     const int nn = n * n;   // n <= 46340
-#pragma unroll 4
-    for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nn; t += gridDim.x * blockDim.x) {
         if (t % (n + 1) != 0) a = fmax(a, fabs(S[t]));   // off the diagonal
         if (R) b = fmax(b, fabs(R[t]));
         if (hnorm) h = fmax(h, fabs(H[t]));
@@ -157,10 +160,29 @@ __global__ void k_rs_norms(const double *__restrict__ S, const double *__restric
             s1 = fmax(s1, red[1][k]);
             s2 = fmax(s2, red[2][k]);
         }
-        st[1] = s0;
-        st[2] = s1;
-        if (hnorm) st[3] = s2;
+        atomic_max_nonneg(st + 1, s0);
+        atomic_max_nonneg(st + 2, s1);
+        if (hnorm) atomic_max_nonneg(st + 3, s2);
     }
+}
+
+__global__ void k_rs_clear_norms(double *st, int hnorm)
+{
+    if (threadIdx.x == 0) {
+        st[1] = 0.0;
+        st[2] = 0.0;
+        if (hnorm) st[3] = 0.0;
+    }
+}
+
+// k_rs_norms over enough workgroups to spread the n^2 entries
+static void rs_norms(const double *S, const double *R, const double *H, int n, int hnorm,
+                     double *st)
+{
+    Context &c = ctx();
+    const int nb = (int)std::min<long>(128, ((long)n * n + 255) / 256);
+    hipLaunchKernelGGL(k_rs_clear_norms, dim3(1), dim3(64), 0, c.stream, st, hnorm);
+    hipLaunchKernelGGL(k_rs_norms, dim3(nb), dim3(256), 0, c.stream, S, R, H, n, hnorm, st);
 }
 
 // Cluster search, one workgroup.  lam_i = B_ii/(1 - R_ii); strong pairs;
@@ -455,7 +477,7 @@ static int refine_eigh(const double *H, int n, double *V, double *D, int *ok)
     const dim3 g2((n + 127) / 128, n), b2(128);
     const double one = 1.0, zero = 0.0;
     RS_HIP(hipMemsetAsync(st, 0, sizeof(double) * 8, c.stream));
-    hipLaunchKernelGGL(k_rs_norms, dim3(1), dim3(1024), 0, c.stream, H, nullptr, H, n, 1, st);
+    rs_norms(H, nullptr, H, n, 1, st);
     hipLaunchKernelGGL(k_set_identity, g2, b2, 0, c.stream, X, n);
     const double *Bc = H, *Rc = nullptr;   // B and R of the current X (X = I)
     double hs[8];
@@ -484,7 +506,7 @@ static int refine_eigh(const double *H, int n, double *V, double *D, int *ok)
         RS_TRY(rs_project(H, X, n, T, B, R));
         Bc = B;
         Rc = R;
-        hipLaunchKernelGGL(k_rs_norms, dim3(1), dim3(1024), 0, c.stream, B, R, H, n, 0, st);
+        rs_norms(B, R, H, n, 0, st);
         RS_HIP(hipGetLastError());
         RS_TRY(read_st(hs, st, 8));
         if (hs[5] != 0.0 || hs[4] != 0.0) return kOK;   // cluster too large / Jacobi stalled
