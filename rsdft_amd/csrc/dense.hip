@@ -185,6 +185,37 @@ static void rs_norms(const double *S, const double *R, const double *H, int n, i
     hipLaunchKernelGGL(k_rs_norms, dim3(nb), dim3(256), 0, c.stream, S, R, H, n, hnorm, st);
 }
 
+// Strong pairs i < j of the cluster search (|B_ij + l_j R_ij| > kStrongPair
+// |l_j - l_i|, l_i = B_ii / (1 - R_ii)) over many workgroups, appended to
+// pairs[0 .. kPairMax) (i) and pairs[kPairMax ..) (j) through the counter
+// pairs[2 kPairMax], which must be zero on entry; the count may exceed
+// kPairMax (then only the first kPairMax are stored).
+__global__ void __launch_bounds__(256)
+k_rs_pairs(const double *__restrict__ B, const double *__restrict__ R, int n,
+           int *__restrict__ pairs)
+{
+    const int nn = n * n;   // n <= 46340
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nn; t += gridDim.x * blockDim.x) {
+        const int j = t / n, i = t - j * n;
+        if (i >= j) continue;
+        const double li = B[(long)i * n + i] / (1.0 - (R ? R[(long)i * n + i] : 0.0));
+        const double lj = B[(long)j * n + j] / (1.0 - (R ? R[(long)j * n + j] : 0.0));
+        const double num = B[t] + lj * (R ? R[t] : 0.0);
+        if (fabs(num) > kStrongPair * fabs(lj - li)) {
+            const int k = atomicAdd(pairs + 2 * kPairMax, 1);
+            if (k < kPairMax) {
+                pairs[k] = i;
+                pairs[kPairMax + k] = j;
+            }
+        }
+    }
+}
+
+__global__ void k_rs_clear_pairs(int *pairs)
+{
+    if (threadIdx.x == 0) pairs[2 * kPairMax] = 0;
+}
+
 // Cluster search, one workgroup.  lam_i = B_ii/(1 - R_ii); strong pairs;
 // label propagation to connected components; compact member lists:
 // cl[0] = number of clusters (size >= 2), cl[1 .. ncl+1] = offsets,
@@ -193,7 +224,8 @@ static void rs_norms(const double *S, const double *R, const double *H, int n, i
 // st[7] = number of strong pairs.  R may be null (R = 0).
 __global__ void __launch_bounds__(1024)
 k_rs_clusters(const double *__restrict__ B, const double *__restrict__ R, int n,
-              double *__restrict__ lam, int *__restrict__ clu, int *__restrict__ cl, double *st)
+              const int *__restrict__ pairs, double *__restrict__ lam, int *__restrict__ clu,
+              int *__restrict__ cl, double *st)
 {
     extern __shared__ unsigned char smem_raw[];
     double *l = reinterpret_cast<double *>(smem_raw);
@@ -210,19 +242,11 @@ k_rs_clusters(const double *__restrict__ B, const double *__restrict__ R, int n,
         label[i] = i;
         cnt[i] = 0;
     }
-    if (tid == 0) { npairs = 0; fail = 0; maxsz = 0; }
+    if (tid == 0) { npairs = pairs[2 * kPairMax]; fail = 0; maxsz = 0; }
     __syncthreads();
-    const int nn = n * n;   // n <= 46340
-#pragma unroll 4
-    for (int t = tid; t < nn; t += nt) {
-        const int j = t / n, i = t - j * n;
-        if (i >= j) continue;
-        const double num = B[t] + l[j] * (R ? R[t] : 0.0);
-        const double gap = l[j] - l[i];
-        if (fabs(num) > kStrongPair * fabs(gap)) {
-            const int k = atomicAdd(&npairs, 1);
-            if (k < kPairMax) { pi[k] = i; pj[k] = j; }
-        }
+    for (int k = tid; k < min(npairs, kPairMax); k += nt) {
+        pi[k] = pairs[k];
+        pj[k] = pairs[kPairMax + k];
     }
     __syncthreads();
     const int np = npairs < kPairMax ? npairs : kPairMax;
@@ -470,10 +494,12 @@ static int refine_eigh(const double *H, int n, double *V, double *D, int *ok)
     const size_t lds = (size_t)n * sizeof(double) + 2 * (size_t)n * sizeof(int) +
                        2 * (size_t)kPairMax * sizeof(int);
     if (lds > 64 * 1024) return kOK;   // cluster search does not fit one workgroup
-    RS_TRY(c.evwork.ensure(sizeof(double) * (6 * nn + n + 8) + sizeof(int) * (3 * n + 4)));
+    RS_TRY(c.evwork.ensure(sizeof(double) * (6 * nn + n + 8) +
+                           sizeof(int) * (3 * n + 4 + 2 * kPairMax + 1)));
     double *X = c.evwork.as<double>(), *T = X + nn, *B = T + nn, *R = B + nn, *F = R + nn,
            *Xn = F + nn, *lam = Xn + nn, *st = lam + n;
     int *clu = reinterpret_cast<int *>(st + 8), *cl = clu + n;
+    int *pairs = cl + 2 * n + 4;   // 2 kPairMax + 1 ints
     const dim3 g2((n + 127) / 128, n), b2(128);
     const double one = 1.0, zero = 0.0;
     RS_HIP(hipMemsetAsync(st, 0, sizeof(double) * 8, c.stream));
@@ -484,8 +510,11 @@ static int refine_eigh(const double *H, int n, double *V, double *D, int *ok)
     bool converged = false;
     for (int it = 0; it < kRefineIters; ++it) {
         // 1-2: exact rotation inside every cluster
-        hipLaunchKernelGGL(k_rs_clusters, dim3(1), dim3(1024), lds, c.stream, Bc, Rc, n, lam, clu,
-                           cl, st);
+        hipLaunchKernelGGL(k_rs_clear_pairs, dim3(1), dim3(64), 0, c.stream, pairs);
+        hipLaunchKernelGGL(k_rs_pairs, dim3((unsigned)std::min<long>(256, ((long)n * n + 255) / 256)),
+                           dim3(256), 0, c.stream, Bc, Rc, n, pairs);
+        hipLaunchKernelGGL(k_rs_clusters, dim3(1), dim3(1024), lds, c.stream, Bc, Rc, n, pairs,
+                           lam, clu, cl, st);
         if (it == 0) {
             // large clusters at the start mean the Ritz vectors are still moving:
             // the tridiagonal eigensolver is cheaper than many refinement steps
