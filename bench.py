@@ -8,13 +8,17 @@ operators, FP64 small dense algebra) seconds per sweep; the FP64 sweep time,
 their ratio and the eigenvalue error of the mixed run (after two FP64
 refinement sweeps) against the FP64 run are reported beside it.
 
-The roofline object is for the dominant kernel of the sweep, timed live here
-with HIP events on the library's stream.  The CPU baseline is the oracle's
-OpenMP FP64 restatement on a bounded sample of the same workload.
+The roofline object is for the dominant operator of the sweep, the FP32
+hamiltonian over all bands (three kernels), timed live here with HIP events on
+the library's stream, against its minimal HBM bytes; each kernel is listed
+with its own bytes.  The sweep roofline (sum of bytes / HBM peak + flops /
+MFMA peak over the sweep's kernels), the GEMMs' MFMA utilisation and the
+accuracy gate (pass/fail at 1e-6 Ha) are reported beside it.  The CPU
+baseline is one sweep of the oracle's OpenMP FP64 restatement.
 
 Usage: python bench.py [--gpus 1] [--steps 9] [--warmup 1]
 
-HBM traffic of the roofline kernel comes from two rocprofv3 counter runs of
+HBM traffic of the hamiltonian's kernels comes from two rocprofv3 counter runs of
 `python bench.py --traffic-probe` (FETCH_SIZE, then WRITE_SIZE), each scaled by
 what the same counter reads for the library's plain copy of all bands; they are
 summarised with `python bench.py --pmc-summary FETCH_DIR WRITE_DIR` into
@@ -34,9 +38,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}    # dense MFMA, MI355X
 # sweeps of the accuracy comparison (the FP64 Si216 run is 10 sweeps; the
 # mixed run replaces the first 8 by mixed-precision sweeps)
 NSWEEP = 10
+NMIXED = 8
+EIG_TOL_HA = 1e-6
 
 
 def log(msg):
@@ -44,29 +51,83 @@ def log(msg):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hpsi_traffic_si216.json")
+# the kernels of one FP32 H psi (rocprofv3 kernel-name patterns)
+HPSI_KERNELS = {
+    "projection": r"k_nl_project<float,",
+    "nonlocal_fill": r"k_nl_fill<float,",
+    "stencil": r"k_hpsi2<float, \d+, \d+, \d+, 11,",
+}
 
 
-def hpsi_stencil_bytes(sysm, nb, s, acc):
-    """Algorithmic HBM bytes of one all-band launch of the stencil kernel:
-    psi read + H psi write per band and Vloc once; accumulating (H psi) also
-    the read of the nonlocal term that the grid-order fill wrote into H psi,
-    one more band array."""
+def sphere_points(sysm):
+    """Nat * P: grid points of all projector spheres (one channel per atom)."""
+    return int(np.sum(sysm.proj.MJJ_MAP[::4]))
+
+
+def covered_points(sysm):
+    """Distinct grid points inside some projector sphere."""
+    pr = sysm.proj
+    return int(np.unique(np.concatenate([pr.JJP[l, :pr.MJJ[l]] for l in range(pr.nzlma)])).size)
+
+
+def hpsi_bytes(sysm, nb, s):
+    """Minimal HBM bytes of one H psi over nb bands, band block = all bands:
+    psi read + H psi write, Vloc once, the projector values (4 channels) and
+    an int32 index per sphere point once:  2 ML nb s + ML s + Nat P (4 s + 4)."""
     ML = sysm.grid.ML
-    b = 2 * ML * nb * s + ML * s
-    if acc:
-        b += ML * nb * s
-    return b
+    return 2 * ML * nb * s + ML * s + sphere_points(sysm) * (4 * s + 4)
 
 
-def nl_fill_bytes(sysm, nb, s=4):
-    """Algorithmic HBM bytes of one all-band launch of the grid-order
-    nonlocal fill: the term written at every grid point of every band."""
-    return sysm.grid.ML * nb * s
+def hpsi_flops(sysm, nb):
+    """57 ML + 16 Nat P per band: stencil 55, Vloc 2, 4 projectors x (project
+    + apply) x 2 per sphere point."""
+    return nb * (57 * sysm.grid.ML + 16 * sphere_points(sysm))
+
+
+def hpsi_kernel_bytes(sysm, nb, s):
+    """Algorithmic bytes of each kernel of one H psi as this library runs it:
+    projection reads psi at the covered points and the tables; the fill writes
+    the nonlocal term at every point; the stencil reads psi and that term,
+    writes H psi, reads Vloc."""
+    ML = sysm.grid.ML
+    tables = sphere_points(sysm) * (4 * s + 4)
+    return {"projection": covered_points(sysm) * nb * s + tables,
+            "nonlocal_fill": ML * nb * s,
+            "stencil": 3 * ML * nb * s + ML * s}
+
+
+def sweep_ledger(sysm, nb, s, reject_frac=0.0):
+    """Algorithmic work of one fused sweep (CG with Ncg = 3 and the s-step
+    preconditioner, then Cholesky + Rayleigh-Ritz): kernel -> (calls, bytes
+    per call, flops per call).  A = ML nb s is one band-sized array."""
+    ML = sysm.grid.ML
+    A = ML * nb * s
+    sym = ML * nb * (nb + 1)          # one triangle of an nb x nb product
+    return {
+        "hamiltonian": (4 + reject_frac, hpsi_bytes(sysm, nb, s), hpsi_flops(sysm, nb)),
+        "cg_first_dots_and_gradient": (1, 5 * A, 4 * ML * nb),
+        "precond_gram_pass": (3, A, 0),
+        "precond_apply_pass": (3, 3 * A - A / 3, 0),     # no old direction at icg = 1
+        "cg_2x2_dots": (3, 4 * A, 12 * ML * nb),
+        "cg_update": (3, 7 * A, 10 * ML * nb),
+        "overlap_xtx": (1, A, sym),
+        "xt_hx": (1, 2 * A, sym),
+        "rotation": (1, 2 * A, 2 * ML * nb * nb),
+    }
+
+
+def ledger_time_ms(ledger, peak_tflops):
+    """Sum over kernels of bytes / HBM peak + flops / MFMA peak."""
+    t = 0.0
+    for calls, b, f in ledger.values():
+        t += calls * (b / (HBM_PEAK_GBS * 1e9) + f / (peak_tflops * 1e12))
+    return t * 1e3
 
 
 def pmc_summary(fetch_dir, write_dir, out_path):
-    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the FP32 hamiltonian stencil
-    kernel, scaled by what the same counters read for the plain copy of all bands."""
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of each FP32 hamiltonian
+    kernel, scaled by what the same counters read for the plain copy of all
+    bands (whose algorithmic bytes are known)."""
     import csv
     import glob
     import re
@@ -83,25 +144,31 @@ def pmc_summary(fetch_dir, write_dir, out_path):
     sysm = make_config("si216")
     ML, nb = sysm.grid.ML, sysm.nband
     copy_bytes = ML * nb * 4            # read (and write) of one FP32 copy of all bands
-    res = {}
+    res = {"kernels": {}}
+    total = 0.0
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         vals = load(d, counter)
-        kern = [k for k in vals if re.search(r"k_hpsi2<float, \d+, \d+, \d+, 11,", k)]
         copy = [k for k in vals if "k_copy_bands<float>" in k]
-        if not kern or not copy:
-            raise SystemExit(f"{counter}: kernels not found in {d}")
-        kv = float(np.mean(vals[kern[0]])) * 1024
+        if not copy:
+            raise SystemExit(f"{counter}: copy kernel not found in {d}")
         cv = float(np.mean(vals[copy[0]])) * 1024
         scale = copy_bytes / cv
-        res[counter] = {"kernel": kern[0], "kernel_bytes_raw": kv, "copy_bytes_raw": cv,
-                        "copy_bytes_algorithmic": copy_bytes, "scale": scale,
-                        "kernel_bytes": kv * scale, "dispatches": len(vals[kern[0]])}
-    res["traffic_bytes"] = res["FETCH_SIZE"]["kernel_bytes"] + res["WRITE_SIZE"]["kernel_bytes"]
-    res["algorithmic_bytes"] = hpsi_stencil_bytes(sysm, nb, 4, True)
+        res[counter + "_scale"] = scale
+        for part, pat in HPSI_KERNELS.items():
+            kern = [k for k in vals if re.search(pat, k)]
+            if not kern:
+                raise SystemExit(f"{counter}: {part} kernel not found in {d}")
+            kv = float(np.mean(vals[kern[0]])) * 1024 * scale
+            e = res["kernels"].setdefault(part, {"kernel": kern[0]})
+            e[counter] = kv
+            e["dispatches"] = len(vals[kern[0]])
+            total += kv
+    res["traffic_bytes"] = total
+    res["algorithmic_bytes"] = hpsi_bytes(sysm, nb, 4)
     res["commands"] = [
-        "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc FETCH_SIZE "
+        "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc FETCH_SIZE "
         "-d <repo>/gpurun_out/pmc_fetch -o run -f csv -- python <repo>/bench.py --traffic-probe",
-        "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc WRITE_SIZE "
+        "cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc WRITE_SIZE "
         "-d <repo>/gpurun_out/pmc_write -o run -f csv -- python <repo>/bench.py --traffic-probe",
         "python bench.py --pmc-summary gpurun_out/pmc_fetch gpurun_out/pmc_write",
     ]
@@ -121,41 +188,39 @@ def traffic_probe():
     dev.attach(x)
     t = dev.test_hpsi(3)
     log(f"probe hpsi ms: {t}")
+    dev.finalize()
 
 
-def cpu_baseline(sysm, x0, nsample):
-    """Oracle FP64 sweep on the host: CG on `nsample` bands (scaled to all
-    bands) plus full-size Gram-Schmidt and subspace diagonalisation."""
+def cpu_baseline(sysm, x0):
+    """One oracle FP64 sweep (C/OpenMP restatement) from x0 on the host cores:
+    CG on all bands, then Gram-Schmidt and subspace diag.  Returns the
+    cpu_baseline object and the oracle's eigenvalues."""
     from oracle.oracle import Oracle
     o = Oracle(sysm)
     nb = x0.shape[0]
-    if nsample <= 0 or nsample > nb:
-        nsample = nb
     t0 = time.perf_counter()
-    o.conjugate_gradient(x0[:nsample])
+    xc, _, _ = o.conjugate_gradient(x0)
     t_cg = time.perf_counter() - t0
-    log(f"cpu: CG {nsample} bands {t_cg:.2f}s")
+    log(f"cpu: CG {nb} bands {t_cg:.2f}s")
     t0 = time.perf_counter()
-    q = o.gram_schmidt(x0)
+    q = o.gram_schmidt(xc)
     t_gs = time.perf_counter() - t0
     log(f"cpu: GS {nb} bands {t_gs:.2f}s")
     t0 = time.perf_counter()
-    o.subspace_diag(q)
+    _, esp, _ = o.subspace_diag(q)
     t_sd = time.perf_counter() - t0
     log(f"cpu: subspace {nb} bands {t_sd:.2f}s")
-    value = t_cg * nb / nsample + t_gs + t_sd
+    value = t_cg + t_gs + t_sd
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": value,
         "unit": "s/sweep",
         "cores": threads,
         "kind": "port",
-        "sample": ((f"oracle FP64 sweep on all {nb} bands (CG, Gram-Schmidt, subspace diag)"
-                    if nsample == nb else
-                    f"oracle FP64 CG on {nsample} of {nb} bands scaled x{nb / nsample:g}, plus "
-                    f"full {nb}-band Gram-Schmidt and subspace diag")
-                   + f"; measured {t_cg + t_gs + t_sd:.1f}s of CPU work"),
-    }
+        "sample": (f"one oracle FP64 sweep of the same workload (CG on all {nb} bands, "
+                   f"Gram-Schmidt, subspace diag) from the same start; "
+                   f"{value:.1f}s of CPU work on {threads} OpenMP threads"),
+    }, esp
 
 
 def main():
@@ -166,8 +231,6 @@ def main():
     ap.add_argument("--workload", default="si216")
     ap.add_argument("--bands", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=0,
-                    help="bands of the CPU CG sample (0: all bands, no scaling)")
     ap.add_argument("--hpsi-reps", type=int, default=20)
     ap.add_argument("--gemm", choices=["fp32", "bf16"], default="fp32",
                     help="GEMM precision of the mixed path")
@@ -190,7 +253,7 @@ def main():
 
     K, W = args.steps, args.warmup
     sysm = make_config(args.workload, nband=args.bands)
-    nb = sysm.nband
+    nb, ML = sysm.nband, sysm.grid.ML
     log(f"system {sysm.name}: grid {sysm.grid.N}, {nb} bands, {sysm.proj.nzlma} projectors")
     t0 = time.perf_counter()
     x = sysm.psi0()
@@ -210,9 +273,10 @@ def main():
     dev.sweep(x, K)
     t_fp64 = (time.perf_counter() - t0) / K
     tim64 = dev.timers()
-    log(f"FP64: {t_fp64 * 1e3:.1f} ms/sweep  {tim64}")
+    log(f"FP64: {t_fp64 * 1e3:.2f} ms/sweep  {tim64}")
     hp64 = dev.test_hpsi(args.hpsi_reps)
-    log(f"FP64 hpsi ms: {hp64}")
+    gm64 = dev.test_gemm(5)
+    log(f"FP64 hpsi ms: {hp64}; gemm ms {gm64}")
     dev.detach()
 
     # ---- mixed sweeps --------------------------------------------------------
@@ -226,12 +290,13 @@ def main():
     dev.sweep(xm, K)
     t_mix = (time.perf_counter() - t0) / K
     timm = dev.timers()
-    log(f"mixed: {t_mix * 1e3:.1f} ms/sweep  {timm}")
+    log(f"mixed: {t_mix * 1e3:.2f} ms/sweep  {timm}")
     hpm = dev.test_hpsi(args.hpsi_reps)
-    log(f"mixed hpsi ms: {hpm}")
+    gmm = dev.test_gemm(5)
+    log(f"mixed hpsi ms: {hpm}; gemm ms {gmm}")
     dev.detach()
 
-    # ---- accuracy: NSWEEP FP64 sweeps against (NSWEEP - 2) mixed + 2 FP64 ----
+    # ---- accuracy: NSWEEP FP64 sweeps against NMIXED mixed + the rest FP64 ----
     def run(start, nmixed, nfp64):
         y = start.copy()
         if nmixed:
@@ -246,10 +311,12 @@ def main():
         dev.detach()
         return esp
 
+    esp64_1 = run(x0, 0, 1)
     esp64 = run(x0, 0, NSWEEP)
-    espm = run(x0, NSWEEP - 2, 2)
+    espm = run(x0, NMIXED, NSWEEP - NMIXED)
     # sensitivity floor: FP64 from the start rounded once to FP32
     espp = run(x0.astype(np.float32).astype(np.float64), 0, NSWEEP)
+    dev.finalize()
     occ = sysm.occ
 
     def errs(e):
@@ -260,32 +327,61 @@ def main():
     flo_err, flo_err_occ, flo_eband = errs(espp)
     d = np.abs(espm - esp64)
     worst = np.argsort(d)[::-1][:5]
-    log(f"after {NSWEEP} sweeps ({NSWEEP - 2} mixed + 2 FP64 vs {NSWEEP} FP64): max eig err "
-        f"{eig_err:.3e} Ha (occupied {eig_err_occ:.3e}), band energy {eband_err:.3e}; worst bands "
-        f"{[int(i) for i in worst]} errs {[f'{d[i]:.2e}' for i in worst]}")
+    log(f"after {NSWEEP} sweeps ({NMIXED} mixed + {NSWEEP - NMIXED} FP64 vs {NSWEEP} FP64): max "
+        f"eig err {eig_err:.3e} Ha (occupied {eig_err_occ:.3e}), band energy {eband_err:.3e}; "
+        f"worst bands {[int(i) for i in worst]} errs {[f'{d[i]:.2e}' for i in worst]}")
     log(f"FP64 from the FP32-rounded start: max {flo_err:.3e} Ha (occupied {flo_err_occ:.3e}), "
         f"band energy {flo_eband:.3e}")
+    t_protocol = (NMIXED * t_mix + (NSWEEP - NMIXED) * t_fp64) / NSWEEP
 
-    # ---- roofline of the dominant kernel (hpsi stencil, FP32) ---------------
-    t_kernel_ms = float(hpm[4])
-    B = hpsi_stencil_bytes(sysm, nb, 4, True)
-    achieved = B / (t_kernel_ms * 1e-3) / 1e9
-    B64 = hpsi_stencil_bytes(sysm, nb, 8, True)
-    ach64 = B64 / (float(hp64[4]) * 1e-3) / 1e9
-    copy_gbs = 2 * sysm.grid.ML * nb * 4 / (float(hpm[7]) * 1e-3) / 1e9
-    fill_gbs = nl_fill_bytes(sysm, nb) / (float(hpm[6]) * 1e-3) / 1e9
+    # ---- roofline: the whole FP32 H psi (its kernels, HIP events) -----------
+    s32 = 4
+    B = hpsi_bytes(sysm, nb, s32)
+    t_h = float(hpm[0])
+    achieved = B / (t_h * 1e-3) / 1e9
+    kb = hpsi_kernel_bytes(sysm, nb, s32)
+    kt = {"projection": float(hpm[5]), "nonlocal_fill": float(hpm[6]), "stencil": float(hpm[4])}
+    per_kernel = {k: {"ms": kt[k], "algorithmic_bytes": kb[k],
+                      "gbs": kb[k] / (kt[k] * 1e-3) / 1e9,
+                      "frac": kb[k] / (kt[k] * 1e-3) / 1e9 / HBM_PEAK_GBS} for k in kt}
+    B64 = hpsi_bytes(sysm, nb, 8)
+    copy_gbs = 2 * ML * nb * s32 / (float(hpm[7]) * 1e-3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        # counters of the kernel timed here (the accumulating FP32 stencil) only
-        if ", 11," in tr["FETCH_SIZE"]["kernel"]:
+        if "kernels" in tr:
             traffic = float(tr["traffic_bytes"])
+            for k, e in tr["kernels"].items():
+                if k in per_kernel:
+                    per_kernel[k]["traffic"] = e["FETCH_SIZE"] + e["WRITE_SIZE"]
 
-    cpu = None
+    # ---- GEMMs on MFMA -------------------------------------------------------
+    sym_flops = ML * nb * (nb + 1)
+    rot_flops = 2 * ML * nb * nb
+
+    def gemm_util(ms, peak):
+        fl = (sym_flops, sym_flops, rot_flops)
+        names = ("overlap_xtx", "xt_hx", "rotation")
+        return {n: {"ms": float(t), "tflops": f / (t * 1e-3) / 1e12,
+                    "mfma_util": f / (t * 1e-3) / 1e12 / peak} for n, t, f in zip(names, ms, fl)}
+
+    # ---- sweep roofline denominators (sum of bytes/BW + flops/peak) ----------
+    rej = float(timm["cg_rejected_bands"]) / nb
+    led32 = sweep_ledger(sysm, nb, 4, rej)
+    led64 = sweep_ledger(sysm, nb, 8, float(tim64["cg_rejected_bands"]) / nb)
+    roof32 = ledger_time_ms(led32, MFMA_PEAK_TFLOPS["f32"])
+    roof64 = ledger_time_ms(led64, MFMA_PEAK_TFLOPS["f64"])
+
+    cpu, esp_oracle = (None, None)
     if not args.no_cpu:
-        cpu = cpu_baseline(sysm, x0, args.cpu_sample)
-        log(f"cpu baseline {cpu['value']:.2f} s/sweep")
+        cpu, esp_oracle = cpu_baseline(sysm, x0)
+        log(f"cpu baseline {cpu['value']:.2f} s/sweep; GPU FP64 first sweep vs oracle: "
+            f"{np.max(np.abs(esp64_1 - esp_oracle)):.3e} Ha")
+
+    def phases(tim):
+        return {k: v / K for k, v in tim.items()
+                if k not in ("hpsi_bands", "eig_refined", "cg_rejected_bands")}
 
     out = {
         "metric": "s/sweep Si216 mixed vs FP64 (@1e-6 Ha eig err); hpsi HBM GB/s vs peak",
@@ -306,35 +402,60 @@ def main():
             "mixed_gemm": args.gemm,
             "fp64_s_per_sweep": t_fp64,
             "fp64_over_mixed": t_fp64 / t_mix,
-            "accuracy_protocol": f"{NSWEEP} FP64 sweeps vs {NSWEEP - 2} mixed + 2 FP64 sweeps from the same start",
-            "max_eig_err_ha": eig_err,
-            "max_eig_err_occupied_ha": eig_err_occ,
-            "band_energy_err_ha": eband_err,
-            "fp64_rounded_start_eig_err_ha": flo_err,
-            "fp64_rounded_start_eig_err_occupied_ha": flo_err_occ,
-            "hpsi_ms_fp32": float(hpm[0]),
+            "accuracy_protocol": f"{NSWEEP} FP64 sweeps vs {NMIXED} mixed + {NSWEEP - NMIXED} FP64 "
+                                 "sweeps from the same orthonormalised seeded start",
+            "protocol_s_per_sweep": t_protocol,
+            "fp64_over_protocol": t_fp64 / t_protocol,
+            "accuracy_gate": {
+                "tol_ha": EIG_TOL_HA,
+                "max_eig_err_ha": eig_err,
+                "max_eig_err_occupied_ha": eig_err_occ,
+                "band_energy_err_ha": eband_err,
+                "all_bands_pass": eig_err <= EIG_TOL_HA,
+                "occupied_bands_pass": eig_err_occ <= EIG_TOL_HA,
+                "band_energy_pass": eband_err <= EIG_TOL_HA,
+                "fp64_rounded_start_eig_err_ha": flo_err,
+                "fp64_rounded_start_eig_err_occupied_ha": flo_err_occ,
+                "fp64_rounded_start_band_energy_err_ha": flo_eband,
+            },
+            "fp64_first_sweep_vs_oracle_max_eig_err_ha":
+                None if esp_oracle is None else float(np.max(np.abs(esp64_1 - esp_oracle))),
+            "hpsi_ms_fp32": t_h,
             "hpsi_ms_fp64": float(hp64[0]),
-            "hpsi_stencil_gbs_fp64": ach64,
-            "nl_fill_gbs_fp32": fill_gbs,
+            "hpsi_gbs_fp64": B64 / (float(hp64[0]) * 1e-3) / 1e9,
+            "hpsi_gflops_fp32": hpsi_flops(sysm, nb) / (t_h * 1e-3) / 1e9,
+            "hpsi_gflops_fp64": hpsi_flops(sysm, nb) / (float(hp64[0]) * 1e-3) / 1e9,
             "hpsi_parts_ms_fp32": {k: float(v) for k, v in zip(
                 ("hamiltonian", "kinetic", "localpot", "nonlocal", "stencil", "projection",
                  "nl_stage", "copy"), hpm)},
             "copy_gbs_fp32": copy_gbs,
-            "phase_ms_mixed": {k: v / K for k, v in timm.items() if k not in ("hpsi_bands", "eig_refined")},
+            "gemm_fp32": gemm_util(gmm, MFMA_PEAK_TFLOPS["f32"]),
+            "gemm_fp64": gemm_util(gm64, MFMA_PEAK_TFLOPS["f64"]),
+            "sweep_roofline": {
+                "mixed_ms": roof32, "mixed_frac": roof32 / (t_mix * 1e3),
+                "fp64_ms": roof64, "fp64_frac": roof64 / (t_fp64 * 1e3),
+                "note": "sum over the sweep's streaming kernels and GEMMs of bytes / 8 TB/s + "
+                        "flops / MFMA peak; the FP64 Cholesky and eigensolve (latency-bound) "
+                        "are not in it",
+            },
+            "phase_ms_mixed": phases(timm),
+            "hpsi_bands_per_sweep_mixed": timm["hpsi_bands"] / K,
             "eig_refined_mixed": timm["eig_refined"],
-            "phase_ms_fp64": {k: v / K for k, v in tim64.items() if k not in ("hpsi_bands", "eig_refined")},
+            "phase_ms_fp64": phases(tim64),
+            "hpsi_bands_per_sweep_fp64": tim64["hpsi_bands"] / K,
             "eig_refined_fp64": tim64["eig_refined"],
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_hpsi2<float,6,2,1,kin|loc|acc,1024> (Md=6 FD stencil + Vloc added to "
-                      "the nonlocal term in H psi, all bands)",
+            "kernel": "hamiltonian, FP32, all 480 bands: k_nl_project + k_nl_fill + "
+                      "k_hpsi2<float,6,2,1,kin|loc|acc,1024> (Md=6 FD stencil + Vloc + KB nonlocal)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "algorithmic_bytes": B,
+            "per_kernel": per_kernel,
         },
         "cpu_baseline": cpu,
     }

@@ -167,11 +167,23 @@ RSDFT_API int rsdft_gpu_test_hpsi(int nrep, double *times_ms);
 RSDFT_API int rsdft_gpu_test_eigh(const double *H, int n, int mode, double *w, double *V,
                                   double *ms);
 
+/* Measurement: device time in ms (mean over nrep) of the tall GEMMs of one
+ * sweep on the attached wavefunctions, in the working precision:
+ * [0] the overlap dV X^T X (gram_schmidt_t_module.f90:187-199),
+ * [1] X^T (H X) as the subspace matrix is formed (one triangle of tiles),
+ * [2] the rotation X U (subspace_rotv_sl_module.f90:24-135).  No reference
+ * counterpart (the reference times whole routines only). */
+RSDFT_API int rsdft_gpu_test_gemm(int nrep, double *times_ms);
+
 /* Per-phase timers in milliseconds since the last reset (watch_module.f90
  * time_hmlt/time_cgpc analogue): [0] hamiltonian, [1] preconditioner,
  * [2] CG total, [3] Gram-Schmidt, [4] subspace_diag, [5] sweep total,
- * [6] number of band-hamiltonian applications, [7] number of subspace
- * eigensolves done by refinement.  n <= 8. */
+ * [6] number of band-hamiltonian applications (bands a masked launch
+ * skips are not counted), [7] number of subspace eigensolves done by
+ * refinement, [8] number of bands whose CG step was rejected
+ * (cg_module.f90:361-364) in the last CG call.  n <= 9.  The hamiltonian
+ * and preconditioner timers run inside the CG, Gram-Schmidt and sweep
+ * timers (they overlap them, as the reference's time_hmlt does). */
 RSDFT_API int rsdft_gpu_get_timers(double *t, int n);
 RSDFT_API int rsdft_gpu_reset_timers(void);
 

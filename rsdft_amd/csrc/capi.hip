@@ -39,41 +39,96 @@ int DevMem::ensure(size_t nbytes)
     return kOK;
 }
 
-static Context g_ctx;
-Context &ctx() { return g_ctx; }
+// The context lives on the heap and is never destroyed: a file-scope object
+// would free device memory from its destructor during static destruction,
+// after the HIP runtime (or a profiler attached to it) has shut down.
+// rsdft_gpu_finalize releases everything while HIP is still up.
+static Context *const g_ctx = new Context();
+Context &ctx() { return *g_ctx; }
 
 // ---- timers ---------------------------------------------------------------
-struct PhaseEvents {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    std::vector<int> slot;
-    int begin(int s) {
-        hipEvent_t a, b;
-        RS_HIP(hipEventCreate(&a));
-        RS_HIP(hipEventCreate(&b));
-        RS_HIP(hipEventRecord(a, ctx().stream));
-        ev.push_back({a, b});
-        slot.push_back(s);
-        return kOK;
+int PhaseEvents::begin(int s, int *handle)
+{
+    hipEvent_t e0, e1;
+    RS_HIP(hipEventCreate(&e0));
+    RS_HIP(hipEventCreate(&e1));
+    RS_HIP(hipEventRecord(e0, ctx().stream));
+    a.push_back(e0);
+    b.push_back(e1);
+    slot.push_back(s);
+    *handle = (int)a.size() - 1;
+    return kOK;
+}
+
+int PhaseEvents::end(int handle)
+{
+    RS_HIP(hipEventRecord(b[handle], ctx().stream));
+    return kOK;
+}
+
+int PhaseEvents::flush()
+{
+    for (size_t k = 0; k < a.size(); ++k) {
+        float ms = 0.0f;
+        RS_HIP(hipEventElapsedTime(&ms, a[k], b[k]));
+        ctx().timers.ms[slot[k]] += ms;
+        (void)hipEventDestroy(a[k]);
+        (void)hipEventDestroy(b[k]);
     }
-    int end() {
-        RS_HIP(hipEventRecord(ev.back().second, ctx().stream));
-        return kOK;
-    }
-    int flush() {   // stream must be synchronised
-        for (size_t k = 0; k < ev.size(); ++k) {
-            float ms = 0.0f;
-            RS_HIP(hipEventElapsedTime(&ms, ev[k].first, ev[k].second));
-            ctx().timers.ms[slot[k]] += ms;
-            (void)hipEventDestroy(ev[k].first);
-            (void)hipEventDestroy(ev[k].second);
+    a.clear();
+    b.clear();
+    slot.clear();
+    return kOK;
+}
+
+int phase_begin(int s, int *handle)
+{
+    *handle = -1;
+    return ctx().pe ? ctx().pe->begin(s, handle) : kOK;
+}
+
+int phase_end(int handle)
+{
+    return (ctx().pe && handle >= 0) ? ctx().pe->end(handle) : kOK;
+}
+
+// Makes `pe` the timing target of the drivers for the lifetime of the scope.
+struct TimedScope {
+    PhaseEvents pe;
+    TimedScope() { ctx().pe = &pe; }
+    ~TimedScope() {
+        ctx().pe = nullptr;
+        for (size_t k = 0; k < pe.a.size(); ++k) {   // left by an error return
+            (void)hipEventDestroy(pe.a[k]);
+            (void)hipEventDestroy(pe.b[k]);
         }
-        ev.clear();
-        slot.clear();
-        return kOK;
     }
 };
 
-enum { kTHam = 0, kTPc = 1, kTCg = 2, kTGs = 3, kTSd = 4, kTSweep = 5 };
+__global__ void k_count_active(const int *__restrict__ act, int nb, unsigned long long *count)
+{
+    __shared__ int tot;
+    if (threadIdx.x == 0) tot = 0;
+    __syncthreads();
+    int n = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) n += act[b] != 0;
+    atomicAdd(&tot, n);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(count, (unsigned long long)tot);
+}
+
+int count_active_bands(const int *band_active, int nb)
+{
+    Context &c = ctx();
+    if (!c.hpsi_count.p) {
+        RS_TRY(c.hpsi_count.ensure(sizeof(unsigned long long)));
+        RS_HIP(hipMemsetAsync(c.hpsi_count.p, 0, sizeof(unsigned long long), c.stream));
+    }
+    hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, c.stream, band_active, nb,
+                       c.hpsi_count.as<unsigned long long>());
+    RS_HIP(hipGetLastError());
+    return kOK;
+}
 
 // ---- helpers --------------------------------------------------------------
 static int need_init()
@@ -169,34 +224,34 @@ static int clear_solver_flags()
 template <class T> static T *psiT() { return ctx().psi.as<T>(); }
 
 template <class T>
-static int run_sweeps(int MB, int nsweep, double *esp_dev, double *res_dev, PhaseEvents &pe)
+static int run_sweeps(int MB, int nsweep, double *esp_dev, double *res_dev)
 {
     for (int it = 0; it < nsweep; ++it) {
-        RS_TRY(pe.begin(kTSweep));
-        RS_TRY(pe.begin(kTCg));
+        int hs, hp;
+        RS_TRY(phase_begin(kTSweep, &hs));
+        RS_TRY(phase_begin(kTCg, &hp));
         RS_TRY(cg_batched<T>(psiT<T>(), MB, esp_dev, res_dev));
-        RS_TRY(pe.end());
+        RS_TRY(phase_end(hp));
         if (ctx().sweep_mode == 1) {
             // CG leaves H x in c.hx except for bands with a rejected step
             // (there H x moved and x did not): recompute those bands only
             T *HX = ctx().hx.as<T>();
-            RS_TRY(pe.begin(kTGs));
+            RS_TRY(phase_begin(kTGs, &hp));
             RS_TRY(hpsi_apply<T>(psiT<T>(), HX, MB, kKin | kLoc | kNl, ctx().cg.rejected));
             RS_TRY(cholesky_inverse_dev<T>(psiT<T>(), MB));
-            RS_TRY(pe.end());
-            RS_TRY(pe.begin(kTSd));
+            RS_TRY(phase_end(hp));
+            RS_TRY(phase_begin(kTSd, &hp));
             RS_TRY(rayleigh_ritz_factored_dev<T>(psiT<T>(), HX, MB, esp_dev));
-            RS_TRY(pe.end());
+            RS_TRY(phase_end(hp));
         } else {
-            RS_TRY(pe.begin(kTGs));
+            RS_TRY(phase_begin(kTGs, &hp));
             RS_TRY(gram_schmidt_dev<T>(psiT<T>(), MB));
-            RS_TRY(pe.end());
-            RS_TRY(pe.begin(kTSd));
+            RS_TRY(phase_end(hp));
+            RS_TRY(phase_begin(kTSd, &hp));
             RS_TRY(subspace_diag_dev<T>(psiT<T>(), MB, esp_dev));
-            RS_TRY(pe.end());
+            RS_TRY(phase_end(hp));
         }
-        // close the sweep pair (it was opened first)
-        RS_HIP(hipEventRecord(pe.ev[pe.ev.size() - 4].second, ctx().stream));
+        RS_TRY(phase_end(hs));
     }
     return kOK;
 }
@@ -650,17 +705,18 @@ int rsdft_gpu_conjugate_gradient(int n1, int n2, int MB, int k, int s, double *u
     if (k != 1 || s != 1) return set_error(kErrArg, "Gamma point, one spin only");
     bool staged = false;
     RS_TRY(bind_wf(unk, n2 - n1 + 1, MB, &staged));
-    PhaseEvents pe;
+    TimedScope ts;
     double *ed = c.dots.as<double>();
-    RS_TRY(pe.begin(kTCg));
+    int h;
+    RS_TRY(phase_begin(kTCg, &h));
     if (c.prec == 0) RS_TRY(cg_batched<double>(psiT<double>(), MB, ed, ed + MB));
     else RS_TRY(cg_batched<float>(psiT<float>(), MB, ed, ed + MB));
-    RS_TRY(pe.end());
+    RS_TRY(phase_end(h));
     RS_HIP(hipMemcpyAsync(esp, ed, sizeof(double) * MB, hipMemcpyDeviceToHost, c.stream));
     RS_HIP(hipMemcpyAsync(res, ed + MB, sizeof(double) * MB, hipMemcpyDeviceToHost, c.stream));
     if (staged) RS_TRY(download_psi(unk, MB));
     RS_HIP(hipStreamSynchronize(c.stream));
-    return pe.flush();
+    return ts.pe.flush();
 }
 
 int rsdft_gpu_gram_schmidt(double *unk, int ML0, int MB, int n0, int n1)
@@ -672,14 +728,15 @@ int rsdft_gpu_gram_schmidt(double *unk, int ML0, int MB, int n0, int n1)
     bool staged = false;
     RS_TRY(bind_wf(unk, ML0, MB, &staged));
     RS_TRY(clear_solver_flags());
-    PhaseEvents pe;
-    RS_TRY(pe.begin(kTGs));
+    TimedScope ts;
+    int h;
+    RS_TRY(phase_begin(kTGs, &h));
     if (c.prec == 0) RS_TRY(gram_schmidt_dev<double>(psiT<double>(), MB));
     else RS_TRY(gram_schmidt_dev<float>(psiT<float>(), MB));
-    RS_TRY(pe.end());
+    RS_TRY(phase_end(h));
     if (staged) RS_TRY(download_psi(unk, MB));
     RS_HIP(hipStreamSynchronize(c.stream));
-    RS_TRY(pe.flush());
+    RS_TRY(ts.pe.flush());
     return check_solver_flags();
 }
 
@@ -690,16 +747,17 @@ int rsdft_gpu_subspace_diag(double *unk, double *esp, int ML0, int MB)
     bool staged = false;
     RS_TRY(bind_wf(unk, ML0, MB, &staged));
     RS_TRY(clear_solver_flags());
-    PhaseEvents pe;
+    TimedScope ts;
     double *ed = c.dots.as<double>();
-    RS_TRY(pe.begin(kTSd));
+    int h;
+    RS_TRY(phase_begin(kTSd, &h));
     if (c.prec == 0) RS_TRY(subspace_diag_dev<double>(psiT<double>(), MB, ed));
     else RS_TRY(subspace_diag_dev<float>(psiT<float>(), MB, ed));
-    RS_TRY(pe.end());
+    RS_TRY(phase_end(h));
     RS_HIP(hipMemcpyAsync(esp, ed, sizeof(double) * MB, hipMemcpyDeviceToHost, c.stream));
     if (staged) RS_TRY(download_psi(unk, MB));
     RS_HIP(hipStreamSynchronize(c.stream));
-    RS_TRY(pe.flush());
+    RS_TRY(ts.pe.flush());
     return check_solver_flags();
 }
 
@@ -710,15 +768,15 @@ int rsdft_gpu_sweep(double *unk, double *esp, double *res, int ML0, int MB, int 
     bool staged = false;
     RS_TRY(bind_wf(unk, ML0, MB, &staged));
     RS_TRY(clear_solver_flags());
-    PhaseEvents pe;
+    TimedScope ts;
     double *ed = c.dots.as<double>();
-    if (c.prec == 0) RS_TRY(run_sweeps<double>(MB, nsweep, ed, ed + MB, pe));
-    else RS_TRY(run_sweeps<float>(MB, nsweep, ed, ed + MB, pe));
+    if (c.prec == 0) RS_TRY(run_sweeps<double>(MB, nsweep, ed, ed + MB));
+    else RS_TRY(run_sweeps<float>(MB, nsweep, ed, ed + MB));
     if (esp) RS_HIP(hipMemcpyAsync(esp, ed, sizeof(double) * MB, hipMemcpyDeviceToHost, c.stream));
     if (res) RS_HIP(hipMemcpyAsync(res, ed + MB, sizeof(double) * MB, hipMemcpyDeviceToHost, c.stream));
     if (staged) RS_TRY(download_psi(unk, MB));
     RS_HIP(hipStreamSynchronize(c.stream));
-    RS_TRY(pe.flush());
+    RS_TRY(ts.pe.flush());
     return check_solver_flags();
 }
 
@@ -841,6 +899,15 @@ int rsdft_gpu_test_hpsi(int nrep, double *times_ms)
     return kOK;
 }
 
+int rsdft_gpu_test_gemm(int nrep, double *times_ms)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    if (!c.host_unk) return set_error(kErrState, "attach wavefunctions first");
+    if (!times_ms) return set_error(kErrArg, "null times");
+    return time_gemms(nrep, times_ms);
+}
+
 int rsdft_gpu_test_eigh(const double *H, int n, int mode, double *w, double *V, double *ms)
 {
     RS_TRY(need_init());
@@ -879,10 +946,27 @@ int rsdft_gpu_test_eigh(const double *H, int n, int mode, double *w, double *V, 
 int rsdft_gpu_get_timers(double *t, int n)
 {
     Context &c = ctx();
-    if (n > 8) n = 8;
+    if (n > 9) n = 9;
     for (int k = 0; k < n && k < 6; ++k) t[k] = c.timers.ms[k];
-    if (n > 6) t[6] = (double)c.hpsi_bands;
+    if (n > 6) {
+        unsigned long long masked = 0;
+        if (c.hpsi_count.p) {
+            RS_HIP(hipStreamSynchronize(c.stream));
+            RS_HIP(hipMemcpy(&masked, c.hpsi_count.p, sizeof(masked), hipMemcpyDeviceToHost));
+        }
+        t[6] = (double)c.hpsi_bands + (double)masked;
+    }
     if (n > 7) t[7] = (double)c.eig_refined;
+    if (n > 8) {   // bands whose CG step was rejected in the last CG call
+        t[8] = 0.0;
+        if (c.cg.nb > 0 && c.cg.flags.p) {
+            std::vector<int> rej(c.cg.nb);
+            RS_HIP(hipStreamSynchronize(c.stream));
+            RS_HIP(hipMemcpy(rej.data(), c.cg.rejected, sizeof(int) * c.cg.nb,
+                             hipMemcpyDeviceToHost));
+            for (int r : rej) t[8] += r != 0;
+        }
+    }
     return kOK;
 }
 
@@ -892,6 +976,8 @@ int rsdft_gpu_reset_timers(void)
     for (double &x : c.timers.ms) x = 0.0;
     c.hpsi_bands = 0;
     c.eig_refined = 0;
+    if (c.hpsi_count.p)
+        RS_HIP(hipMemsetAsync(c.hpsi_count.p, 0, sizeof(unsigned long long), c.stream));
     return kOK;
 }
 
@@ -906,7 +992,7 @@ int rsdft_gpu_finalize(void)
                      &c.cg.buf, &c.cg.flags, &c.nl.g_lma, &c.nl.g_ptr, &c.nl.g_idx, &c.nl.g_val64,
                      &c.nl.g_val32, &c.nl.g_order, &c.nl.scale64, &c.nl.d_ptr, &c.nl.d_idx, &c.nl.d_val,
                      &c.nl.fill_rptr, &c.nl.fill_rgrp, &c.nl.fill_meta, &c.nl.fill_uv, &c.nl.fill_uv64,
-                     &c.nl.amap_dev};
+                     &c.nl.amap_dev, &c.hpsi_count};
     for (DevMem *m : all) m->release();
     if (c.blas) rocblas_destroy_handle(c.blas);
     if (c.stream) (void)hipStreamDestroy(c.stream);

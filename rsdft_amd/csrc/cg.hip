@@ -503,6 +503,7 @@ int CGScalars::ensure(int MB)
     accept = f + MB;
     pc_active = f + 2 * (size_t)MB;
     rejected = f + 3 * (size_t)MB;
+    nb = MB;
     return kOK;
 }
 
@@ -617,15 +618,21 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
     for (int icg = 1; icg <= c.ncg + 1; ++icg) {
         hipLaunchKernelGGL(k_cg_top, bgrid, bblk, 0, c.stream, v, nb);
         if (icg == c.ncg + 1) break;
+        int hpc;
+        RS_TRY(phase_begin(kTPc, &hpc));
         if (sstep) {
             // Gram pass -> d, (Pg,g) -> rb, beta -> P = Pg + beta P in the apply pass
+            // (the preconditioner timer covers both passes and so also the
+            // direction update fused into the second)
             double *dcoef = c.pc_coef.as<double>(), *pgg = dcoef + (size_t)nb * c.mloop;
             RS_TRY(precond_sstep_coeffs<T>(G, nb, act, part, dcoef, pgg));
             hipLaunchKernelGGL(k_cg_pgg_gram, bgrid, bblk, 0, c.stream, v, pgg, nb, dV,
                                icg == 1 ? 1 : 0);
             RS_TRY(precond_sstep_apply<T>(G, P, nb, act, dcoef, 1, icg == 1 ? nullptr : v.bk));
+            RS_TRY(phase_end(hpc));
         } else {
             RS_TRY(precond_batched<T>(G, PG, nb, act));
+            RS_TRY(phase_end(hpc));
             {
                 const T *a[1] = {PG}, *bb[1] = {G};
                 RS_TRY(band_dots<T>(1, a, bb, nb, 1.0, nullptr, act));

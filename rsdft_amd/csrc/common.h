@@ -140,11 +140,26 @@ struct CGScalars {
     double *E, *E1, *rb, *res, *gkgk, *bk, *u1, *u2, *W, *rr0, *rr1, *pa, *pb;
     int *active, *accept, *pc_active;
     int *rejected;    // 1 when a step of this band was rejected in the last CG call
+    int nb = 0;       // bands of the last CG call
     int ensure(int MB);
 };
 
 struct Timers {
     double ms[8] = {};
+};
+
+// Timer slots of rsdft_gpu_get_timers (the reference's watch_module groups:
+// hamiltonian, preconditioner, CG, Gram-Schmidt, subspace diag, whole sweep)
+enum { kTHam = 0, kTPc = 1, kTCg = 2, kTGs = 3, kTSd = 4, kTSweep = 5 };
+
+// HIP event pairs on the library stream, summed into Timers at flush().
+// begin() returns a handle that end() closes, so phases may nest.
+struct PhaseEvents {
+    std::vector<hipEvent_t> a, b;
+    std::vector<int> slot;
+    int begin(int s, int *handle);
+    int end(int handle);
+    int flush();   // the stream must be synchronised
 };
 
 struct Context {
@@ -182,12 +197,20 @@ struct Context {
     DevMem pc_coef;    // s-step preconditioner coefficients [band][mloop]
     CGScalars cg;
     Timers timers;
-    long hpsi_bands = 0;
+    PhaseEvents *pe = nullptr;   // events of the running entry point, null: untimed
+    long hpsi_bands = 0;         // bands of unmasked H psi launches
+    DevMem hpsi_count;           // device counter: bands of masked H psi launches
     // BF16 split-GEMM scratch
     DevMem gemm_part;
 };
 
 Context &ctx();
+
+// Phase timing from inside the drivers: no-ops when no entry point is timing.
+int phase_begin(int slot, int *handle);
+int phase_end(int handle);
+// Adds the number of set flags of band_active[0..nb) to the masked H psi count.
+int count_active_bands(const int *band_active, int nb);
 
 // ---- launchers (implemented in the .hip files) ----------------------------
 // geometry
@@ -262,6 +285,10 @@ template <class T>
 int cholesky_inverse_dev(const T *x, int nb);
 template <class T>
 int rayleigh_ritz_factored_dev(T *x, const T *hx, int nb, double *esp_dev);
+
+// device time (ms) of the tall GEMMs of a sweep on the attached psi:
+// [0] x^T x, [1] x^T y (symmetric product), [2] x U
+int time_gemms(int nrep, double *ms);
 
 // adjoint
 template <class T>

@@ -490,6 +490,7 @@ static int refine_eigh(const double *H, int n, double *V, double *D, int *ok)
 {
     Context &c = ctx();
     *ok = 0;
+    if (n < 2) return kOK;   // nothing to refine (and no pair for the Jacobi launch)
     const size_t nn = (size_t)n * n;
     const size_t lds = (size_t)n * sizeof(double) + 2 * (size_t)n * sizeof(int) +
                        2 * (size_t)kPairMax * sizeof(int);
@@ -732,6 +733,46 @@ int rayleigh_ritz_factored_dev(T *X, const T *HX, int nb, double *esp_dev)
     std::swap(c.psi.p, c.tmp.p);
     std::swap(c.psi.bytes, c.tmp.bytes);
     return kOK;
+}
+
+// Device time (ms, mean over nrep after one untimed call) of the sweep's tall
+// GEMMs on the attached psi (x) and a band-sized work array (y) in the working
+// precision: [0] S = dV x^T x, [1] C = x^T y (symmetric product: upper tile
+// pairs), [2] out = x U with U = I (nb x nb FP64).
+template <class T>
+static int time_gemms_t(int nrep, double *ms)
+{
+    Context &c = ctx();
+    const int nb = c.MB;
+    RS_TRY(ensure_dense(nb));
+    T *X = c.psi.as<T>(), *Y = c.hp.as<T>(), *Out = c.tmp.as<T>();
+    double *S = c.mat.as<double>(), *C = c.mat2.as<double>(), *U = c.mat3.as<double>();
+    hipLaunchKernelGGL(k_set_identity, dim3((nb + 127) / 128, nb), dim3(128), 0, c.stream, U, nb);
+    RS_HIP(hipGetLastError());
+    hipEvent_t a, b;
+    RS_HIP(hipEventCreate(&a));
+    RS_HIP(hipEventCreate(&b));
+    for (int t = 0; t < 3; ++t) {
+        for (int rep = -1; rep < nrep; ++rep) {
+            if (rep == 0) RS_HIP(hipEventRecord(a, c.stream));
+            if (t == 0) RS_TRY(gemm_tn<T>(X, X, nb, c.g.dV, S));
+            else if (t == 1) RS_TRY(gemm_tn<T>(X, Y, nb, 1.0, C, true));
+            else RS_TRY(gemm_nn<T>(X, U, nb, Out, 1));
+        }
+        RS_HIP(hipEventRecord(b, c.stream));
+        RS_HIP(hipEventSynchronize(b));
+        float t_ms = 0.0f;
+        RS_HIP(hipEventElapsedTime(&t_ms, a, b));
+        ms[t] = nrep > 0 ? t_ms / nrep : 0.0;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return kOK;
+}
+
+int time_gemms(int nrep, double *ms)
+{
+    return ctx().prec == 0 ? time_gemms_t<double>(nrep, ms) : time_gemms_t<float>(nrep, ms);
 }
 
 template int convert_from_f64<double>(const double *, double *, long);

@@ -685,10 +685,9 @@ static int nl_fill(int nb, const T *coef, const int *band_active, T *out)
 static bool fill_ready() { return ctx().nl.nz > 0 && ctx().nl.fill_mm > 0; }
 
 template <class T>
-int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
+static int hpsi_apply_untimed(const T *in, T *out, int nb, int flags, const int *band_active)
 {
     Context &c = ctx();
-    if (nb <= 0) return kOK;
     if (c.nl.nz <= 0) flags &= ~kNl;
     const int sflags = flags & (kKin | kLoc | kAcc);
     if (flags & kNl) {
@@ -699,9 +698,7 @@ int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
     // accumulates kinetic + local onto it
     if (sflags == (kKin | kLoc) && (flags & kNl) && fill_ready()) {
         RS_TRY(nl_fill<T>(nb, c.coefC.as<T>(), band_active, out));
-        RS_TRY(launch_stencil<T>(in, out, nb, kKin | kLoc | kAcc, band_active));
-        c.hpsi_bands += nb;
-        return kOK;
+        return launch_stencil<T>(in, out, nb, kKin | kLoc | kAcc, band_active);
     }
     if (sflags & (kKin | kLoc)) {
         RS_TRY(launch_stencil<T>(in, out, nb, sflags, band_active));
@@ -709,7 +706,20 @@ int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
         RS_HIP(hipMemsetAsync(out, 0, sizeof(T) * (size_t)c.g.ML * nb, c.stream));
     }
     if (flags & kNl) RS_TRY(nl_apply<T>(out, nb, c.coefC.as<T>(), band_active));
-    c.hpsi_bands += nb;
+    return kOK;
+}
+
+template <class T>
+int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
+{
+    if (nb <= 0) return kOK;
+    int h;
+    RS_TRY(phase_begin(kTHam, &h));
+    RS_TRY(hpsi_apply_untimed<T>(in, out, nb, flags, band_active));
+    RS_TRY(phase_end(h));
+    // bands applied: all nb, or those the mask selects (counted on the device)
+    if (band_active) RS_TRY(count_active_bands(band_active, nb));
+    else ctx().hpsi_bands += nb;
     return kOK;
 }
 
@@ -727,11 +737,19 @@ int nl_stage(T *out, int nb, const T *coef)
     return nl_apply<T>(out, nb, coef, nullptr);
 }
 
+// Plain copy of nb bands with 16-byte vector loads and stores: the stream
+// rate a kernel that reads one array and writes one attains on this device.
 template <class T>
 __global__ void __launch_bounds__(256) k_copy_bands(const T *__restrict__ in, T *__restrict__ out, long n)
 {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-         i += (long)gridDim.x * blockDim.x)
+    using V = uint4;   // 16 bytes
+    constexpr int E = 16 / sizeof(T);
+    const long nv = n / E;
+    const V *vi = reinterpret_cast<const V *>(in);
+    V *vo = reinterpret_cast<V *>(out);
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nv; i += stride) vo[i] = vi[i];
+    for (long i = nv * E + blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += stride)
         out[i] = in[i];
 }
 
@@ -740,6 +758,8 @@ int copy_bands(const T *in, T *out, int nb)
 {
     Context &c = ctx();
     const long n = c.g.ML * nb;
+    if ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15)
+        return set_error(kErrArg, "copy_bands: 16-byte aligned arrays expected");
     hipLaunchKernelGGL((k_copy_bands<T>), dim3(8192), dim3(256), 0, c.stream, in, out, n);
     RS_HIP(hipGetLastError());
     return kOK;

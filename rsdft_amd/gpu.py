@@ -15,6 +15,7 @@ There is no CPU fallback: a missing library raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 
@@ -38,7 +39,7 @@ EXPORTS = (
     "rsdft_gpu_wf_sync", "rsdft_gpu_wf_detach", "rsdft_gpu_conjugate_gradient",
     "rsdft_gpu_gram_schmidt", "rsdft_gpu_subspace_diag", "rsdft_gpu_sweep",
     "rsdft_gpu_calc_density", "rsdft_gpu_force_ps_nloc2", "rsdft_gpu_band_energy_grad",
-    "rsdft_gpu_test_hpsi", "rsdft_gpu_test_eigh", "rsdft_gpu_get_timers", "rsdft_gpu_reset_timers",
+    "rsdft_gpu_test_hpsi", "rsdft_gpu_test_gemm", "rsdft_gpu_test_eigh", "rsdft_gpu_get_timers", "rsdft_gpu_reset_timers",
     "rsdft_gpu_finalize", "rsdft_gpu_last_error",
 )
 
@@ -85,6 +86,7 @@ def load_library(path: str = LIB_PATH):
         "rsdft_gpu_band_energy_grad": [P, I, I, P, P, P, P],
         "rsdft_gpu_test_eigh": [P, I, I, P, P, P],
         "rsdft_gpu_test_hpsi": [I, P],
+        "rsdft_gpu_test_gemm": [I, P],
         "rsdft_gpu_get_timers": [P, I],
         "rsdft_gpu_reset_timers": [],
         "rsdft_gpu_finalize": [],
@@ -96,6 +98,9 @@ def load_library(path: str = LIB_PATH):
     lib.rsdft_gpu_last_error.argtypes = []
     lib.rsdft_gpu_last_error.restype = ctypes.c_char_p
     _lib = lib
+    # release device memory, the stream and the rocBLAS handle while the HIP
+    # runtime is still up (Python's exit handlers run before the C runtime's)
+    atexit.register(lib.rsdft_gpu_finalize)
     return lib
 
 
@@ -168,6 +173,9 @@ class RsdftGpu:
         MJJ, JJP, uVk, iuV = self._tabs
         self._call("rsdft_gpu_set_projectors", pr.nzlma, pr.MAXMJJ, _p(MJJ), _p(JJP), _p(uVk),
                    _p(iuV))
+        if pr.nzlma == 0:            # no nonlocal part (and no forces)
+            self.natom = 0
+            return
         if with_derivatives:
             self._dtabs = (_i32(pr.amap), _i32(pr.MJJ_MAP), _i32(pr.JJ_MAP), _f64(pr.duVdR))
             amap, mjm, jjm, dud = self._dtabs
@@ -292,11 +300,19 @@ class RsdftGpu:
         self._call("rsdft_gpu_test_hpsi", nrep, _p(t))
         return t
 
+    def test_gemm(self, nrep: int = 5):
+        """ms per call of the sweep's tall GEMMs on the attached psi:
+        X^T X, X^T Y (one triangle), X U."""
+        t = np.zeros(3)
+        self._call("rsdft_gpu_test_gemm", nrep, _p(t))
+        return t
+
     def timers(self):
-        t = np.zeros(8)
-        self._call("rsdft_gpu_get_timers", _p(t), 8)
+        t = np.zeros(9)
+        self._call("rsdft_gpu_get_timers", _p(t), 9)
         return {"hamiltonian": t[0], "precond": t[1], "cg": t[2], "gram_schmidt": t[3],
-                "subspace_diag": t[4], "sweep": t[5], "hpsi_bands": t[6], "eig_refined": t[7]}
+                "subspace_diag": t[4], "sweep": t[5], "hpsi_bands": t[6], "eig_refined": t[7],
+                "cg_rejected_bands": t[8]}
 
     def reset_timers(self):
         self._call("rsdft_gpu_reset_timers")

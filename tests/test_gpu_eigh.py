@@ -109,3 +109,58 @@ def test_tridiagonal_solver_mixed_clusters(dev):
     H = 0.5 * (H + H.T)
     w, V, ms = dev.test_eigh(H, mode=3)
     check(H, w, V)
+
+
+def nearly_diagonal(n, seed, coupling=1e-4, ncluster=20, csize=3, csplit=1e-6):
+    """Ritz values spread over 2 Ha with clusters of csize values csplit apart,
+    plus a symmetric coupling of size `coupling`: the Hsub of bands that are
+    close to converged, the case the refinement eigensolver (mode 1) is for."""
+    rng = np.random.default_rng(seed)
+    d = np.sort(rng.uniform(-0.5, 1.5, n))
+    cand = np.arange(0, n - csize, 2 * csize)
+    starts = rng.choice(cand, min(ncluster, len(cand) // 2), replace=False)
+    for s in starts:
+        d[s:s + csize] = d[s] + csplit * np.arange(csize)
+    b = rng.standard_normal((n, n)) * coupling
+    return np.diag(d) + 0.5 * (b + b.T)
+
+
+@pytest.mark.parametrize("n", [480, 96])
+def test_refinement_nearly_diagonal(dev, n):
+    """Mode 1 on a nearly diagonal Hsub with near-degenerate clusters: the
+    refinement path solves it (counted in the timers) and matches dsyevd."""
+    print(f"case: test_eigh mode 1 nearly diagonal n={n}", flush=True)
+    H = nearly_diagonal(n, seed=n)
+    dev.reset_timers()
+    w, V, ms = dev.test_eigh(H, mode=1)
+    used = dev.timers()["eig_refined"]
+    print(f"  {ms:.3f} ms, refined {used:.0f}", flush=True)
+    assert used == 1
+    check(H, w, V)
+
+
+def test_refinement_pair_overflow_falls_back(dev):
+    """More strong pairs than the cluster search holds (a block of 72 coupled
+    near-degenerate values: 2556 pairs > 2048): mode 1 must fall back to the
+    tridiagonal solver and still return the right eigenpairs."""
+    print("case: test_eigh mode 1 strong-pair overflow n=480", flush=True)
+    n = 480
+    H = nearly_diagonal(n, seed=3, ncluster=0)
+    H[100:172, 100:172] += 1e-3 * np.random.default_rng(4).standard_normal((72, 72))
+    H[100:172, 100:172] = 0.5 * (H[100:172, 100:172] + H[100:172, 100:172].T)
+    np.fill_diagonal(H[100:172, 100:172], H[100, 100])
+    dev.reset_timers()
+    w, V, ms = dev.test_eigh(H, mode=1)
+    used = dev.timers()["eig_refined"]
+    print(f"  {ms:.3f} ms, refined {used:.0f}", flush=True)
+    assert used == 0
+    check(H, w, V)
+
+
+@pytest.mark.parametrize("mode", [1, 0, 3])
+def test_eigensolver_n1_and_n2(dev, mode):
+    """The smallest subspaces: n = 1 and n = 2."""
+    print(f"case: test_eigh mode {mode} n=1,2", flush=True)
+    for H in (np.array([[0.7]]), np.array([[0.3, 0.1], [0.1, -0.2]])):
+        w, V, ms = dev.test_eigh(H, mode=mode)
+        check(H, w, V)
