@@ -167,6 +167,47 @@ RSDFT_API int rsdft_gpu_test_hpsi(int nrep, double *times_ms);
 RSDFT_API int rsdft_gpu_test_eigh(const double *H, int n, int mode, double *w, double *V,
                                   double *ms);
 
+/* ---- on-disk formats (Fortran sequential unformatted, one process,
+ * periodic grid N1 x N2 x N3, LL2 = natural order with 0-based indices).
+ * None of these needs rsdft_gpu_init, except that an unk which is the
+ * attached array is written from / read into its device mirror. ---- */
+
+/* simple_wf_io_write (io_write_module.f90:19-264), IO_ctrl = 0, real
+ * wavefunctions, Gamma, one spin: OC 1, 3 real(8) / 4, 5 real(4) records in
+ * the new format (-1 marker, sizes, MK MS MMBZ, IO_ctrl OC TYPE_WF, LL2, occ,
+ * aa bb kbb, np(0:7) and the four distributions); OC 11, 13 the old format.
+ * Bands MBwr1..MBwr2 (all when out of range, as the reference resets them).
+ * aa, bb: 3x3 column-major (may be NULL: zeros). */
+RSDFT_API int rsdft_gpu_write_wf(const char *file, int N1, int N2, int N3, const double *unk,
+                                 int ML0, int MB, int OC, int MBwr1, int MBwr2,
+                                 const double *occ, const double *aa, const double *bb);
+
+/* simple_wf_io_read (io_read_module.f90:15-521), SYStype = 0, IO_ctrl = 0:
+ * either format, real(8) or real(4) records, values mapped through the
+ * file's LL2.  unk(ML0, MB): bands beyond the file's range are left as they
+ * are, file bands beyond MB are skipped; occ (may be NULL) gets
+ * min(MB, file MB) occupations; *mb_file (may be NULL) the file's MB.  A
+ * different grid, k-points, spins or complex data are refused. */
+RSDFT_API int rsdft_gpu_read_wf(const char *file, int N1, int N2, int N3, double *unk, int ML0,
+                                int MB, double *occ, int *mb_file);
+
+/* write_data's density/potential file (io_module.f90:96-200), versioned
+ * format: version, sizes, LL2, lattice, atoms (aa_atom(3,natom),
+ * ki_atom(natom), zn_atom(nelement)), Fermi energy, then per spin rho, Vloc,
+ * (spin 1) Vh, Vxc.  rho, vloc, vxc: (ML0, MSP); vh: (ML0). */
+RSDFT_API int rsdft_gpu_write_vrho(const char *file, int N1, int N2, int N3, int ML0, int MSP,
+                                   const double *rho, const double *vloc, const double *vh,
+                                   const double *vxc, const double *aa, int nelement, int natom,
+                                   const double *aa_atom, const int *ki_atom, const int *zn_atom,
+                                   double efermi);
+
+/* read_data's density/potential part (io_module.f90:286-409): versioned and
+ * unversioned files; any output may be NULL (skipped); *efermi = 0 for an
+ * unversioned file. */
+RSDFT_API int rsdft_gpu_read_vrho(const char *file, int N1, int N2, int N3, int ML0, int MSP,
+                                  double *rho, double *vloc, double *vh, double *vxc,
+                                  double *efermi);
+
 /* Measurement: device time in ms (mean over nrep) of the tall GEMMs of one
  * sweep on the attached wavefunctions, in the working precision:
  * [0] the overlap dV X^T X (gram_schmidt_t_module.f90:187-199),

@@ -41,6 +41,7 @@ EXPORTS = (
     "rsdft_gpu_calc_density", "rsdft_gpu_force_ps_nloc2", "rsdft_gpu_band_energy_grad",
     "rsdft_gpu_test_hpsi", "rsdft_gpu_test_gemm", "rsdft_gpu_test_eigh", "rsdft_gpu_get_timers", "rsdft_gpu_reset_timers",
     "rsdft_gpu_finalize", "rsdft_gpu_last_error",
+    "rsdft_gpu_write_wf", "rsdft_gpu_read_wf", "rsdft_gpu_write_vrho", "rsdft_gpu_read_vrho",
 )
 
 
@@ -90,6 +91,10 @@ def load_library(path: str = LIB_PATH):
         "rsdft_gpu_get_timers": [P, I],
         "rsdft_gpu_reset_timers": [],
         "rsdft_gpu_finalize": [],
+        "rsdft_gpu_write_wf": [ctypes.c_char_p, I, I, I, P, I, I, I, I, I, P, P, P],
+        "rsdft_gpu_read_wf": [ctypes.c_char_p, I, I, I, P, I, I, P, P],
+        "rsdft_gpu_write_vrho": [ctypes.c_char_p, I, I, I, I, I, P, P, P, P, P, I, I, P, P, P, D],
+        "rsdft_gpu_read_vrho": [ctypes.c_char_p, I, I, I, I, I, P, P, P, P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -114,6 +119,70 @@ def _f64(a):
 
 def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _check(lib, name, r):
+    if r != 0:
+        msg = lib.rsdft_gpu_last_error().decode(errors="replace")
+        raise RsdftGpuError(f"{name} -> {r}: {msg}")
+
+
+def write_wf(path, N, unk, occ, OC=1, MBwr=(1, 0), aa=None, bb=None):
+    """simple_wf_io_write (io_write_module.f90:19-264) of unk (MB, ML) on the
+    periodic grid N = (N1, N2, N3)."""
+    lib = load_library()
+    u = unk if (unk.dtype == np.float64 and unk.flags.c_contiguous) else _f64(unk)
+    occ = _f64(occ)
+    a = None if aa is None else np.asfortranarray(aa, dtype=np.float64)
+    b = None if bb is None else np.asfortranarray(bb, dtype=np.float64)
+    _check(lib, "write_wf", lib.rsdft_gpu_write_wf(
+        os.fsencode(path), N[0], N[1], N[2], _p(u), u.shape[1], u.shape[0], int(OC),
+        int(MBwr[0]), int(MBwr[1]), _p(occ), _p(a), _p(b)))
+
+
+def read_wf(path, N, MB, unk=None):
+    """simple_wf_io_read (io_read_module.f90:15-521): returns (unk (MB, ML),
+    occ (MB,), MB of the file); unk may be given (e.g. the attached array)."""
+    lib = load_library()
+    ML = N[0] * N[1] * N[2]
+    if unk is None:
+        unk = np.zeros((MB, ML))
+    occ = np.zeros(MB)
+    mbf = ctypes.c_int(0)
+    _check(lib, "read_wf", lib.rsdft_gpu_read_wf(
+        os.fsencode(path), N[0], N[1], N[2], _p(unk), ML, MB, _p(occ), ctypes.byref(mbf)))
+    return unk, occ, mbf.value
+
+
+def write_vrho(path, N, rho, vloc, vh, vxc, aa=None, aa_atom=None, ki_atom=None, zn_atom=None,
+               efermi=0.0):
+    """write_data's density/potential file (io_module.f90:96-200); rho, vloc,
+    vxc (MSP, ML), vh (ML,)."""
+    lib = load_library()
+    rho, vloc, vxc = (np.atleast_2d(_f64(a)) for a in (rho, vloc, vxc))
+    vh = _f64(vh)
+    natom = 0 if aa_atom is None else len(ki_atom)
+    nel = 0 if zn_atom is None else len(zn_atom)
+    a = None if aa is None else np.asfortranarray(aa, dtype=np.float64)
+    at = None if aa_atom is None else _f64(aa_atom)
+    ki = None if ki_atom is None else _i32(ki_atom)
+    zn = None if zn_atom is None else _i32(zn_atom)
+    _check(lib, "write_vrho", lib.rsdft_gpu_write_vrho(
+        os.fsencode(path), N[0], N[1], N[2], rho.shape[1], rho.shape[0], _p(rho), _p(vloc),
+        _p(vh), _p(vxc), _p(a), nel, natom, _p(at), _p(ki), _p(zn), float(efermi)))
+
+
+def read_vrho(path, N, MSP=1):
+    """read_data's density/potential part (io_module.f90:286-409): returns
+    (rho (MSP, ML), vloc (MSP, ML), vh (ML,), vxc (MSP, ML), efermi)."""
+    lib = load_library()
+    ML = N[0] * N[1] * N[2]
+    rho, vloc, vxc = (np.zeros((MSP, ML)) for _ in range(3))
+    vh = np.zeros(ML)
+    ef = np.zeros(1)
+    _check(lib, "read_vrho", lib.rsdft_gpu_read_vrho(
+        os.fsencode(path), N[0], N[1], N[2], ML, MSP, _p(rho), _p(vloc), _p(vh), _p(vxc), _p(ef)))
+    return rho, vloc, vh, vxc, float(ef[0])
 
 
 class RsdftGpu:
