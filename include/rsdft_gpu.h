@@ -95,6 +95,17 @@ RSDFT_API int rsdft_gpu_set_sweep_mode(int mode);
 RSDFT_API int rsdft_gpu_hamiltonian(int k, int s, const double *tpsi, double *htpsi,
                           int n1, int n2, int ib1, int ib2);
 
+/* Allocation-free hamiltonian: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same operator as
+ * rsdft_gpu_hamiltonian, always in FP64, on DEVICE arrays d_tpsi / d_htpsi
+ * (ML x nb, band-major); the projection coefficients live in a static 16 MiB
+ * device array per GPU, so nb * nzlma must not exceed 2,097,152 (else
+ * kErrArg).  Calls no hipMalloc/hipFree; synchronises the library stream
+ * before returning.  Calls that overlap on one GPU share that array and race. */
+RSDFT_API int rsdft_gpu_hamiltonian_noalloc(int k, int s, const double *d_tpsi, double *d_htpsi,
+                          int n1, int n2, int ib1, int ib2);
+
 /* op_kinetic (kinetic_module.f90:186-202 -> kinetic_sol_module.f90:20-149):
  * OVERWRITES htpsi. */
 RSDFT_API int rsdft_gpu_op_kinetic(int k, const double *tpsi, double *htpsi,

@@ -130,6 +130,28 @@ int count_active_bands(const int *band_active, int nb)
     return kOK;
 }
 
+// ---- scratch of the allocation-free entry point -----------------------------
+// Device memory of rsdft_gpu_hamiltonian_noalloc: the projection coefficients
+// [lma][band] in FP64.  Every GPU holds its own copy at its own address until
+// the process ends; host code reaches it only through the address looked up
+// per device at rsdft_gpu_init (the array's name is a host-side stand-in).
+constexpr size_t kScratchElems = size_t(2) << 20;   // 16 MiB of FP64
+constexpr int kScratchMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchElems];
+static double *g_scratch_ptr[kScratchMaxDevices] = {};
+
+static int bind_scratch(int device)
+{
+    if (device < 0 || device >= kScratchMaxDevices)
+        return set_error(kErrState, "device %d beyond the %d devices the scratch table holds", device,
+                         kScratchMaxDevices);
+    if (g_scratch_ptr[device]) return kOK;
+    void *p = nullptr;
+    RS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return kOK;
+}
+
 // ---- helpers --------------------------------------------------------------
 static int need_init()
 {
@@ -285,6 +307,7 @@ int rsdft_gpu_init(int N1, int N2, int N3, const double *Hgrid, double dV, int M
         RS_BLAS(rocblas_set_stream(c.blas, c.stream));
         RS_BLAS(rocblas_set_atomics_mode(c.blas, rocblas_atomics_not_allowed));
     }
+    RS_TRY(bind_scratch(c.device));
     c.g = GridDesc();
     c.g.N1 = N1;
     c.g.N2 = N2;
@@ -649,6 +672,30 @@ int rsdft_gpu_hamiltonian(int k, int s, const double *tpsi, double *htpsi, int n
                           int ib1, int ib2)
 {
     return op_entry(k, s, tpsi, htpsi, n1, n2, ib1, ib2, kKin | kLoc | kNl);
+}
+
+int rsdft_gpu_hamiltonian_noalloc(int k, int s, const double *d_tpsi, double *d_htpsi, int n1,
+                                  int n2, int ib1, int ib2)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    if (k != 1) return set_error(kErrArg, "k=%d: Gamma point only", k);
+    if (s != 1) return set_error(kErrArg, "s=%d: one spin channel only", s);
+    const long ML0 = (long)n2 - n1 + 1;
+    const int nb = ib2 - ib1 + 1;
+    if (ML0 != c.g.ML) return set_error(kErrArg, "n2-n1+1=%ld but the grid has %ld points", ML0, c.g.ML);
+    if (nb <= 0) return kOK;
+    if (!d_tpsi || !d_htpsi) return set_error(kErrArg, "null device array");
+    const size_t need = (size_t)nb * std::max(c.nl.nz, 0);
+    if (need > kScratchElems)
+        return set_error(kErrArg,
+                         "%d bands x %d projectors = %zu coefficients exceed the %zu of the "
+                         "allocation-free scratch", nb, c.nl.nz, need, kScratchElems);
+    double *coef = g_scratch_ptr[c.device];
+    if (!coef) return set_error(kErrState, "scratch not bound on device %d", c.device);
+    RS_TRY(hpsi_apply<double>(d_tpsi, d_htpsi, nb, kKin | kLoc | kNl, nullptr, coef));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    return kOK;
 }
 
 int rsdft_gpu_op_kinetic(int k, const double *tpsi, double *htpsi, int n1, int n2, int ib1, int ib2)

@@ -222,8 +222,11 @@ StencilGeom choose_hpsi_geom(int N1, int N2, int N3, int md, int px, int pad, in
 
 // Hamiltonian pieces. flags bit0 kinetic, bit1 local, bit2 nonlocal, bit3 accumulate.
 enum { kKin = 1, kLoc = 2, kNl = 4, kAcc = 8 };
+// coef: caller-supplied room for nb * nz projection coefficients (null: the
+// context's buffer, grown as needed)
 template <class T>
-int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active = nullptr);
+int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active = nullptr,
+               T *coef = nullptr);
 template <class T>
 int nl_project(const T *in, int nb, T *coef, const int *band_active = nullptr);
 // The pieces of one H psi, for timing: the stencil kernel as hpsi_apply runs

@@ -684,20 +684,24 @@ static int nl_fill(int nb, const T *coef, const int *band_active, T *out)
 
 static bool fill_ready() { return ctx().nl.nz > 0 && ctx().nl.fill_mm > 0; }
 
+// coef: room for nb * nz projection coefficients supplied by the caller, or
+// null to use (and grow) the context's buffer
 template <class T>
-static int hpsi_apply_untimed(const T *in, T *out, int nb, int flags, const int *band_active)
+static int hpsi_apply_untimed(const T *in, T *out, int nb, int flags, const int *band_active,
+                              T *coef)
 {
     Context &c = ctx();
     if (c.nl.nz <= 0) flags &= ~kNl;
     const int sflags = flags & (kKin | kLoc | kAcc);
-    if (flags & kNl) {
+    if ((flags & kNl) && !coef) {
         RS_TRY(c.coefC.ensure((size_t)nb * c.nl.nz * sizeof(T)));
-        RS_TRY(nl_project<T>(in, nb, c.coefC.as<T>(), band_active));
+        coef = c.coefC.as<T>();
     }
+    if (flags & kNl) RS_TRY(nl_project<T>(in, nb, coef, band_active));
     // H psi: the nonlocal term written for every grid point, then the stencil
     // accumulates kinetic + local onto it
     if (sflags == (kKin | kLoc) && (flags & kNl) && fill_ready()) {
-        RS_TRY(nl_fill<T>(nb, c.coefC.as<T>(), band_active, out));
+        RS_TRY(nl_fill<T>(nb, coef, band_active, out));
         return launch_stencil<T>(in, out, nb, kKin | kLoc | kAcc, band_active);
     }
     if (sflags & (kKin | kLoc)) {
@@ -705,17 +709,17 @@ static int hpsi_apply_untimed(const T *in, T *out, int nb, int flags, const int 
     } else if (!(flags & kAcc)) {
         RS_HIP(hipMemsetAsync(out, 0, sizeof(T) * (size_t)c.g.ML * nb, c.stream));
     }
-    if (flags & kNl) RS_TRY(nl_apply<T>(out, nb, c.coefC.as<T>(), band_active));
+    if (flags & kNl) RS_TRY(nl_apply<T>(out, nb, coef, band_active));
     return kOK;
 }
 
 template <class T>
-int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active)
+int hpsi_apply(const T *in, T *out, int nb, int flags, const int *band_active, T *coef)
 {
     if (nb <= 0) return kOK;
     int h;
     RS_TRY(phase_begin(kTHam, &h));
-    RS_TRY(hpsi_apply_untimed<T>(in, out, nb, flags, band_active));
+    RS_TRY(hpsi_apply_untimed<T>(in, out, nb, flags, band_active, coef));
     RS_TRY(phase_end(h));
     // bands applied: all nb, or those the mask selects (counted on the device)
     if (band_active) RS_TRY(count_active_bands(band_active, nb));
@@ -785,8 +789,8 @@ int pc_stencil(const T *in, T *Q, T *R, T *PP, T *PG, int nb, int mode, const in
     return kOK;
 }
 
-template int hpsi_apply<double>(const double *, double *, int, int, const int *);
-template int hpsi_apply<float>(const float *, float *, int, int, const int *);
+template int hpsi_apply<double>(const double *, double *, int, int, const int *, double *);
+template int hpsi_apply<float>(const float *, float *, int, int, const int *, float *);
 template int hpsi_stencil_only<double>(const double *, double *, int);
 template int hpsi_stencil_only<float>(const float *, float *, int);
 template int nl_stage<double>(double *, int, const double *);

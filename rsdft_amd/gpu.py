@@ -34,7 +34,7 @@ EXPORTS = (
     "rsdft_gpu_set_projector_derivatives", "rsdft_gpu_set_precision",
     "rsdft_gpu_set_solver_params", "rsdft_gpu_set_precond_mode", "rsdft_gpu_set_eigensolver",
     "rsdft_gpu_set_sweep_mode",
-    "rsdft_gpu_hamiltonian", "rsdft_gpu_op_kinetic",
+    "rsdft_gpu_hamiltonian", "rsdft_gpu_hamiltonian_noalloc", "rsdft_gpu_op_kinetic",
     "rsdft_gpu_op_localpot", "rsdft_gpu_op_nonlocal", "rsdft_gpu_wf_attach",
     "rsdft_gpu_wf_sync", "rsdft_gpu_wf_detach", "rsdft_gpu_conjugate_gradient",
     "rsdft_gpu_gram_schmidt", "rsdft_gpu_subspace_diag", "rsdft_gpu_sweep",
@@ -72,6 +72,7 @@ def load_library(path: str = LIB_PATH):
         "rsdft_gpu_set_eigensolver": [I],
         "rsdft_gpu_set_sweep_mode": [I],
         "rsdft_gpu_hamiltonian": [I, I, P, P, I, I, I, I],
+        "rsdft_gpu_hamiltonian_noalloc": [I, I, P, P, I, I, I, I],
         "rsdft_gpu_op_kinetic": [I, P, P, I, I, I, I],
         "rsdft_gpu_op_localpot": [I, I, I, P, P],
         "rsdft_gpu_op_nonlocal": [I, I, P, P, I, I, I, I],
@@ -259,6 +260,12 @@ class RsdftGpu:
         h = np.empty_like(t)
         self._call("rsdft_gpu_hamiltonian", 1, 1, _p(t), _p(h), 1, self.ML, 1, t.shape[0])
         return h
+
+    def hamiltonian_noalloc(self, d_tpsi: int, d_htpsi: int, nb: int):
+        """The allocation-free hamiltonian on DEVICE arrays (addresses of nb
+        band-major FP64 bands each), FP64 arithmetic."""
+        self._call("rsdft_gpu_hamiltonian_noalloc", 1, 1, ctypes.c_void_p(d_tpsi),
+                   ctypes.c_void_p(d_htpsi), 1, self.ML, 1, int(nb))
 
     def op_kinetic(self, tpsi):
         t = _f64(np.atleast_2d(tpsi))
