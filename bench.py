@@ -234,6 +234,8 @@ def main():
     ap.add_argument("--hpsi-reps", type=int, default=20)
     ap.add_argument("--gemm", choices=["fp32", "bf16"], default="fp32",
                     help="GEMM precision of the mixed path")
+    ap.add_argument("--pc-mode", type=int, choices=[0, 1], default=1,
+                    help="preconditioner: 1 s-step (default), 0 step-by-step recurrence")
     ap.add_argument("--traffic-probe", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     args = ap.parse_args()
@@ -260,6 +262,7 @@ def main():
     log(f"psi0 generated in {time.perf_counter() - t0:.1f}s")
 
     dev = RsdftGpu(sysm, precision=PREC_FP64)
+    dev.set_precond_mode(args.pc_mode)
     dev.attach(x)
     dev.gram_schmidt(x)
     dev.sync()
@@ -381,7 +384,8 @@ def main():
 
     def phases(tim):
         return {k: v / K for k, v in tim.items()
-                if k not in ("hpsi_bands", "eig_refined", "cg_rejected_bands")}
+                if k not in ("hpsi_bands", "eig_refined", "cg_rejected_bands",
+                             "pc_recurrence_bands")}
 
     out = {
         "metric": "s/sweep Si216 mixed vs FP64 (@1e-6 Ha eig err); hpsi HBM GB/s vs peak",
@@ -440,6 +444,8 @@ def main():
             },
             "phase_ms_mixed": phases(timm),
             "hpsi_bands_per_sweep_mixed": timm["hpsi_bands"] / K,
+            "pc_recurrence_bands_per_sweep_mixed": timm["pc_recurrence_bands"] / K,
+            "pc_recurrence_bands_per_sweep_fp64": tim64["pc_recurrence_bands"] / K,
             "eig_refined_mixed": timm["eig_refined"],
             "phase_ms_fp64": phases(tim64),
             "hpsi_bands_per_sweep_fp64": tim64["hpsi_bands"] / K,

@@ -148,6 +148,46 @@ RSDFT_API int rsdft_gpu_sweep(double *unk, double *esp, double *res, int ML0, in
 /* calc_density, NCPP (density_module.f90:129-136): rho = sum occ psi^2. */
 RSDFT_API int rsdft_gpu_calc_density(const double *unk, int ML0, int MB, const double *occ, double *rho);
 
+/* ---- SCF iteration around the sweep (scf_module.f90:146-366), Nspin = 1,
+ * Gamma point, real wavefunctions ---------------------------------------- */
+
+/* calc_hartree (hartree_module -> hartree_sol_module.f90:19-122): Vh of the
+ * density rho(ML) by FFT Poisson on the G sphere of ggrid_module.f90:55-95,
+ * E_h = dV/2 sum Vh rho (may be null). */
+RSDFT_API int rsdft_gpu_calc_hartree(const double *rho, double *vh, double *E_h);
+
+/* calc_xc for the Perdew-Zunger 81 LDA (xc_ldapz81_module.f90:17-75, MSP=1):
+ * Vxc(ML) and E_xc[3] = (Exc, Ex, Ec) (may be null). */
+RSDFT_API int rsdft_gpu_calc_xc(const double *rho, double *vxc, double *E_xc);
+
+/* calc_fermi with Methfessel-Paxton smearing (fermi_mp_module.f90:96-287),
+ * one spin, one k point: occ(nb) (2 per filled band) and the Fermi level.
+ * Host arithmetic; needs no rsdft_gpu_init. */
+RSDFT_API int rsdft_gpu_calc_fermi(const double *esp, int nb, double nelectron, double ekbt,
+                          int kinteg, double *occ, double *efermi);
+
+/* init_scf / init_mixing (scf_module.f90:100-145, mixing_module.f90:69-140)
+ * for imix = 10 (Pulay on rho): the ionic local potential Vion(ML), the
+ * input density rho(ML), the electron count, mmix, beta_mix, ekbt and the
+ * Methfessel-Paxton order.  Sets the hamiltonian's Vloc = Vion + Vh + Vxc. */
+RSDFT_API int rsdft_gpu_scf_init(const double *vion, const double *rho, double nelectron,
+                          int mmix, double beta, double ekbt, int kinteg);
+
+/* One iteration of the scf loop body (scf_module.f90:163-339 with Ndiag = 1,
+ * second_diag = 2): subspace_diag, conjugate_gradient, gram_schmidt,
+ * esp_calc, calc_fermi, Harris energy, calc_density, calc_hartree, calc_xc,
+ * calc_total_energy, calc_sqerr_mixing, perform_mixing, normalize_density
+ * and the new Vloc.  unk(ML0,MB) in/out (attached or staged), esp/occ/res(MB)
+ * out, info[10] = Etot, Ehwf, Eeig, Eloc, Eion, E_H, E_xc, Efermi,
+ * sqerr_rho, sqerr_v. */
+RSDFT_API int rsdft_gpu_scf_iterate(double *unk, int ML0, int MB, double *esp, double *occ,
+                          double *res, double *info);
+
+/* The SCF fields after the last step: rho, Vloc, Vh, Vxc (ML each; any may be
+ * null) -- the module arrays density_module::rho, localpot_module::Vloc,
+ * hartree_variables::Vh, xc_module::Vxc. */
+RSDFT_API int rsdft_gpu_scf_get_fields(double *rho, double *vloc, double *vh, double *vxc);
+
 /* calc_force_ps_nloc2 for real psi on one rank (ps_nloc2_module.f90:1281-1811):
  * force2(3, natom) = -dE/dR of the nonlocal band energy at fixed psi. */
 RSDFT_API int rsdft_gpu_force_ps_nloc2(const double *unk, int ML0, int MB, const double *occ,
@@ -233,7 +273,9 @@ RSDFT_API int rsdft_gpu_test_gemm(int nrep, double *times_ms);
  * [6] number of band-hamiltonian applications (bands a masked launch
  * skips are not counted), [7] number of subspace eigensolves done by
  * refinement, [8] number of bands whose CG step was rejected
- * (cg_module.f90:361-364) in the last CG call.  n <= 9.  The hamiltonian
+ * (cg_module.f90:361-364) in the last CG call, [9] number of band
+ * preconditionings done by the step-by-step recurrence because the s-step
+ * replay could not resolve them (since the last reset).  n <= 10.  The hamiltonian
  * and preconditioner timers run inside the CG, Gram-Schmidt and sweep
  * timers (they overlap them, as the reference's time_hmlt does). */
 RSDFT_API int rsdft_gpu_get_timers(double *t, int n);

@@ -827,6 +827,82 @@ int rsdft_gpu_sweep(double *unk, double *esp, double *res, int ML0, int MB, int 
     return check_solver_flags();
 }
 
+int rsdft_gpu_calc_hartree(const double *rho, double *vh, double *E_h)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    if (!rho || !vh) return set_error(kErrArg, "null rho or vh");
+    const size_t n = (size_t)c.g.ML;
+    RS_TRY(c.stage.ensure(2 * n * sizeof(double)));
+    double *rd = c.stage.as<double>(), *vd = rd + n;
+    RS_HIP(hipMemcpyAsync(rd, rho, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    double eh = 0.0;
+    RS_TRY(hartree_dev(rd, vd, &eh));
+    RS_HIP(hipMemcpyAsync(vh, vd, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    if (E_h) *E_h = eh;
+    return kOK;
+}
+
+int rsdft_gpu_calc_xc(const double *rho, double *vxc, double *E_xc)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    if (!rho || !vxc) return set_error(kErrArg, "null rho or vxc");
+    const size_t n = (size_t)c.g.ML;
+    RS_TRY(c.stage.ensure(2 * n * sizeof(double)));
+    double *rd = c.stage.as<double>(), *vd = rd + n;
+    RS_HIP(hipMemcpyAsync(rd, rho, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    double e3[3];
+    RS_TRY(xc_dev(rd, vd, e3));
+    RS_HIP(hipMemcpyAsync(vxc, vd, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    if (E_xc) for (int k = 0; k < 3; ++k) E_xc[k] = e3[k];
+    return kOK;
+}
+
+int rsdft_gpu_calc_fermi(const double *esp, int nb, double nelectron, double ekbt, int kinteg,
+                         double *occ, double *efermi)
+{
+    if (!esp || !occ || !efermi) return set_error(kErrArg, "null argument");
+    return calc_fermi_host(esp, nb, nelectron, ekbt, kinteg, occ, efermi);
+}
+
+int rsdft_gpu_scf_init(const double *vion, const double *rho, double nelectron, int mmix,
+                       double beta, double ekbt, int kinteg)
+{
+    RS_TRY(need_init());
+    return scf_init(vion, rho, nelectron, mmix, beta, ekbt, kinteg);
+}
+
+int rsdft_gpu_scf_iterate(double *unk, int ML0, int MB, double *esp, double *occ, double *res,
+                          double *info)
+{
+    RS_TRY(need_init());
+    Context &c = ctx();
+    if (!scf_ready()) return set_error(kErrState, "rsdft_gpu_scf_init has not been called");
+    if (!esp || !occ || !info) return set_error(kErrArg, "null esp, occ or info");
+    bool staged = false;
+    RS_TRY(bind_wf(unk, ML0, MB, &staged));
+    RS_TRY(clear_solver_flags());
+    TimedScope ts;
+    double *ed = c.dots.as<double>();
+    if (c.prec == 0) RS_TRY(scf_iterate_dev<double>(MB, ed, ed + MB, esp, occ, info));
+    else RS_TRY(scf_iterate_dev<float>(MB, ed, ed + MB, esp, occ, info));
+    if (res) RS_HIP(hipMemcpyAsync(res, ed + MB, sizeof(double) * MB, hipMemcpyDeviceToHost, c.stream));
+    if (staged) RS_TRY(download_psi(unk, MB));
+    RS_HIP(hipStreamSynchronize(c.stream));
+    RS_TRY(ts.pe.flush());
+    return check_solver_flags();
+}
+
+int rsdft_gpu_scf_get_fields(double *rho, double *vloc, double *vh, double *vxc)
+{
+    RS_TRY(need_init());
+    if (!scf_ready()) return set_error(kErrState, "rsdft_gpu_scf_init has not been called");
+    return scf_fields(rho, vloc, vh, vxc);
+}
+
 int rsdft_gpu_calc_density(const double *unk, int ML0, int MB, const double *occ, double *rho)
 {
     RS_TRY(need_init());
@@ -993,7 +1069,7 @@ int rsdft_gpu_test_eigh(const double *H, int n, int mode, double *w, double *V, 
 int rsdft_gpu_get_timers(double *t, int n)
 {
     Context &c = ctx();
-    if (n > 9) n = 9;
+    if (n > 10) n = 10;
     for (int k = 0; k < n && k < 6; ++k) t[k] = c.timers.ms[k];
     if (n > 6) {
         unsigned long long masked = 0;
@@ -1014,6 +1090,14 @@ int rsdft_gpu_get_timers(double *t, int n)
             for (int r : rej) t[8] += r != 0;
         }
     }
+    if (n > 9) {   // band preconditionings by the recurrence (s-step replay unresolved)
+        unsigned long long fb = 0;
+        if (c.pc_fallback.p) {
+            RS_HIP(hipStreamSynchronize(c.stream));
+            RS_HIP(hipMemcpy(&fb, c.pc_fallback.p, sizeof(fb), hipMemcpyDeviceToHost));
+        }
+        t[9] = (double)fb;
+    }
     return kOK;
 }
 
@@ -1025,6 +1109,8 @@ int rsdft_gpu_reset_timers(void)
     c.eig_refined = 0;
     if (c.hpsi_count.p)
         RS_HIP(hipMemsetAsync(c.hpsi_count.p, 0, sizeof(unsigned long long), c.stream));
+    if (c.pc_fallback.p)
+        RS_HIP(hipMemsetAsync(c.pc_fallback.p, 0, sizeof(unsigned long long), c.stream));
     return kOK;
 }
 
@@ -1039,8 +1125,9 @@ int rsdft_gpu_finalize(void)
                      &c.cg.buf, &c.cg.flags, &c.nl.g_lma, &c.nl.g_ptr, &c.nl.g_idx, &c.nl.g_val64,
                      &c.nl.g_val32, &c.nl.g_order, &c.nl.scale64, &c.nl.d_ptr, &c.nl.d_idx, &c.nl.d_val,
                      &c.nl.fill_rptr, &c.nl.fill_rgrp, &c.nl.fill_meta, &c.nl.fill_uv, &c.nl.fill_uv64,
-                     &c.nl.amap_dev, &c.hpsi_count};
+                     &c.nl.amap_dev, &c.hpsi_count, &c.pc_fallback};
     for (DevMem *m : all) m->release();
+    (void)scf_release();
     if (c.blas) rocblas_destroy_handle(c.blas);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.blas = nullptr;

@@ -198,12 +198,12 @@ __device__ __host__ inline void sygv2(const double *A, const double *B, double *
 // ---------------------------------------------------------------------------
 struct CGView {
     double *E, *E1, *rb, *res, *gkgk, *bk, *u1, *u2, *W, *rr0, *rr1, *pa, *pb;
-    int *active, *accept, *pc_active, *rejected;
+    int *active, *accept, *pc_active, *rejected, *pc_ill, *pc_ok;
 };
 
 static CGView view(CGScalars &s) {
     return CGView{s.E, s.E1, s.rb, s.res, s.gkgk, s.bk, s.u1, s.u2, s.W, s.rr0, s.rr1,
-                  s.pa, s.pb, s.active, s.accept, s.pc_active, s.rejected};
+                  s.pa, s.pb, s.active, s.accept, s.pc_active, s.rejected, s.pc_ill, s.pc_ok};
 }
 
 // E = dV <x|Hx>, E1 = 1e10 (cg_module.f90:162,176-182)
@@ -235,22 +235,24 @@ __global__ void k_cg_top(CGView v, int nb)
     else if (fabs(v.E[b] - v.E1[b]) < 1.0e-15) v.active[b] = 0;
 }
 
-// rb = dV <Pg|g>; beta (cg_module.f90:237-261)
-__global__ void k_cg_pgg(CGView v, const double *part, int nb, int nchunk, double dV, int first)
+// rb = dV <Pg|g>; beta (cg_module.f90:237-261), for the bands mask selects
+__global__ void k_cg_pgg(CGView v, const double *part, int nb, int nchunk, double dV, int first,
+                         const int *mask)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb || !v.active[b]) return;
+    if (b >= nb || !mask[b]) return;
     const double rb = dV * part[(long)b * nchunk];
     v.rb[b] = rb;
     v.bk[b] = first ? 0.0 : rb / v.gkgk[b];
     v.gkgk[b] = rb;
 }
 
-// the same from (Pg, g) given by the s-step preconditioner's Gram matrix
+// the same from (Pg, g) given by the s-step preconditioner's Gram matrix,
+// for the bands it resolved
 __global__ void k_cg_pgg_gram(CGView v, const double *pgg, int nb, double dV, int first)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb || !v.active[b]) return;
+    if (b >= nb || !v.pc_ok[b]) return;
     const double rb = dV * pgg[b];
     v.rb[b] = rb;
     v.bk[b] = first ? 0.0 : rb / v.gkgk[b];
@@ -280,6 +282,29 @@ __global__ void k_cg_2x2(CGView v, const double *part, int nb, int nchunk, doubl
     v.E[b] = W[0];
 }
 
+// split the active bands by whether the s-step replay resolved them; the
+// unresolved ones are counted: count[0] cumulative, count[1] this call
+__global__ void k_cg_split_ill(CGView v, int nb, unsigned long long *count)
+{
+    __shared__ int tot;
+    if (threadIdx.x == 0) tot = 0;
+    __syncthreads();
+    int n = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const int a = v.active[b];
+        const int ill = a && v.pc_ill[b];
+        v.pc_ill[b] = ill;
+        v.pc_ok[b] = a && !ill;
+        n += ill;
+    }
+    atomicAdd(&tot, n);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (tot) atomicAdd(count, (unsigned long long)tot);
+        count[1] = (unsigned long long)tot;
+    }
+}
+
 // new residual norm and the reject rule (cg_module.f90:349-364)
 __global__ void k_cg_reject(CGView v, const double *part, int nb, int nchunk, double dV)
 {
@@ -305,11 +330,11 @@ __global__ void k_cg_final(CGView v, int nb, double *esp, double *res)
 }
 
 // preconditioner scalars (cgpc_hprsdft_module.f90:165-249), no dV
-__global__ void k_pc_init(CGView v, const double *part, int nb, int nchunk)
+__global__ void k_pc_init(CGView v, const double *part, int nb, int nchunk, const int *mask)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    if (!v.active[b]) { v.pc_active[b] = 0; return; }
+    if (!mask[b]) { v.pc_active[b] = 0; return; }
     const double rr1 = sum_parts(part + (long)b * nchunk, nchunk);
     v.rr1[b] = rr1;
     v.pc_active[b] = rr1 < 1.0e-24 ? 0 : 1;
@@ -370,10 +395,10 @@ k_cg_grad(const T *HX, const T *X, T *G, CGView v, long ML, int nchunk, double *
 // p = Pg (first) or p = Pg + beta p
 template <class T>
 __global__ void __launch_bounds__(kChunkThreads)
-k_cg_newp(const T *PG, T *P, CGView v, long ML, int first)
+k_cg_newp(const T *PG, T *P, CGView v, long ML, int first, const int *mask)
 {
     BAND_LOOP_BEGIN
-    if (!v.active[b]) return;
+    if (!mask[b]) return;
     const T bk = (T)v.bk[b];
     if (first) {
         for (long i = i0 + threadIdx.x; i < i1; i += kChunkThreads) P[off + i] = PG[off + i];
@@ -494,7 +519,7 @@ __global__ void k_copy_active(const T *src, T *dst, long ML, const int *active)
 int CGScalars::ensure(int MB)
 {
     RS_TRY(buf.ensure(sizeof(double) * 13 * (size_t)MB));
-    RS_TRY(flags.ensure(sizeof(int) * 4 * (size_t)MB));
+    RS_TRY(flags.ensure(sizeof(int) * 6 * (size_t)MB));
     double *d = buf.as<double>();
     double **slots[13] = {&E, &E1, &rb, &res, &gkgk, &bk, &u1, &u2, &W, &rr0, &rr1, &pa, &pb};
     for (int k = 0; k < 13; ++k) *slots[k] = d + (size_t)k * MB;
@@ -503,6 +528,8 @@ int CGScalars::ensure(int MB)
     accept = f + MB;
     pc_active = f + 2 * (size_t)MB;
     rejected = f + 3 * (size_t)MB;
+    pc_ill = f + 4 * (size_t)MB;
+    pc_ok = f + 5 * (size_t)MB;
     nb = MB;
     return kOK;
 }
@@ -556,18 +583,12 @@ int precond_batched(const T *g, T *pg, int nb, const int *active)
         RS_HIP(hipGetLastError());
         return kOK;
     }
-    if (c.pc_mode == 1 && c.mloop <= 4) {
-        RS_TRY(c.part.ensure(sizeof(double) * std::max(precond_sstep_part_elems(nb),
-                                                        (size_t)6 * nb * nch)));
-        RS_TRY(c.pc_coef.ensure(sizeof(double) * (size_t)nb * c.mloop));
-        return precond_sstep<T>(g, pg, nb, active, c.part.as<double>(), c.pc_coef.as<double>());
-    }
     T *R = c.rk.as<T>(), *PP = c.ppc.as<T>(), *Q = c.qpc.as<T>();
     const int tiles = c.geom_pc.tiles;
     RS_TRY(c.part.ensure(sizeof(double) * (size_t)6 * nb * (nch > tiles ? nch : tiles)));
     double *part = c.part.as<double>();
     RS_TRY(pc_stencil<T>(g, Q, R, PP, pg, nb, 0, active, part));
-    hipLaunchKernelGGL(k_pc_init, bgrid, bblk, 0, c.stream, v, part, nb, tiles);
+    hipLaunchKernelGGL(k_pc_init, bgrid, bblk, 0, c.stream, v, part, nb, tiles, active);
     for (int il = 1; il <= c.mloop; ++il) {
         RS_TRY(pc_stencil<T>(PP, Q, R, PP, pg, nb, 1, c.cg.pc_active, part));
         hipLaunchKernelGGL(k_pc_alpha, bgrid, bblk, 0, c.stream, v, part, nb, tiles);
@@ -595,6 +616,10 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
     RS_TRY(c.part.ensure(std::max(sizeof(double) * (size_t)6 * nb * maxparts,
                                   sizeof(double) * precond_sstep_part_elems(nb))));
     RS_TRY(c.pc_coef.ensure(sizeof(double) * (size_t)nb * (std::max(c.mloop, 1) + 1)));
+    if (!c.pc_fallback.p) {
+        RS_TRY(c.pc_fallback.ensure(2 * sizeof(unsigned long long)));
+        RS_HIP(hipMemsetAsync(c.pc_fallback.p, 0, 2 * sizeof(unsigned long long), c.stream));
+    }
     // the s-step preconditioner hands (Pg, g) and the new direction over directly
     const bool sstep = c.pc_mode == 1 && c.mloop >= 1 && c.mloop <= 4;
     double *part = c.part.as<double>();
@@ -620,28 +645,41 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
         if (icg == c.ncg + 1) break;
         int hpc;
         RS_TRY(phase_begin(kTPc, &hpc));
+        // Pg by the step-by-step recurrence, (Pg, g), beta and P = Pg + beta P
+        // for the bands `mask` selects
+        auto stepwise = [&](const int *mask) -> int {
+            RS_TRY(precond_batched<T>(G, PG, nb, mask));
+            const T *a[1] = {PG}, *bb[1] = {G};
+            RS_TRY(band_dots<T>(1, a, bb, nb, 1.0, nullptr, mask));
+            collapse_parts(part, nb, nch);
+            hipLaunchKernelGGL(k_cg_pgg, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV,
+                               icg == 1 ? 1 : 0, mask);
+            hipLaunchKernelGGL((k_cg_newp<T>), egrid, eblk, 0, c.stream, PG, P, v, ML,
+                               icg == 1 ? 1 : 0, mask);
+            return kOK;
+        };
         if (sstep) {
             // Gram pass -> d, (Pg,g) -> rb, beta -> P = Pg + beta P in the apply pass
             // (the preconditioner timer covers both passes and so also the
-            // direction update fused into the second)
+            // direction update fused into the second); bands whose replay the
+            // Gram forms cannot resolve take the recurrence
             double *dcoef = c.pc_coef.as<double>(), *pgg = dcoef + (size_t)nb * c.mloop;
-            RS_TRY(precond_sstep_coeffs<T>(G, nb, act, part, dcoef, pgg));
+            RS_TRY(precond_sstep_coeffs<T>(G, nb, act, part, dcoef, pgg, v.pc_ill));
+            hipLaunchKernelGGL(k_cg_split_ill, dim3(1), dim3(256), 0, c.stream, v, nb,
+                               c.pc_fallback.as<unsigned long long>());
             hipLaunchKernelGGL(k_cg_pgg_gram, bgrid, bblk, 0, c.stream, v, pgg, nb, dV,
                                icg == 1 ? 1 : 0);
-            RS_TRY(precond_sstep_apply<T>(G, P, nb, act, dcoef, 1, icg == 1 ? nullptr : v.bk));
+            RS_TRY(precond_sstep_apply<T>(G, P, nb, v.pc_ok, dcoef, 1, icg == 1 ? nullptr : v.bk));
+            // the recurrence's launches only when some band needs them
+            unsigned long long nill = 0;
+            RS_HIP(hipMemcpyAsync(&nill, c.pc_fallback.as<unsigned long long>() + 1, sizeof(nill),
+                                  hipMemcpyDeviceToHost, c.stream));
+            RS_HIP(hipStreamSynchronize(c.stream));
+            if (nill) RS_TRY(stepwise(v.pc_ill));
             RS_TRY(phase_end(hpc));
         } else {
-            RS_TRY(precond_batched<T>(G, PG, nb, act));
+            RS_TRY(stepwise(act));
             RS_TRY(phase_end(hpc));
-            {
-                const T *a[1] = {PG}, *bb[1] = {G};
-                RS_TRY(band_dots<T>(1, a, bb, nb, 1.0, nullptr, act));
-                collapse_parts(part, nb, nch);
-                hipLaunchKernelGGL(k_cg_pgg, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV,
-                                   icg == 1 ? 1 : 0);
-            }
-            hipLaunchKernelGGL((k_cg_newp<T>), egrid, eblk, 0, c.stream, PG, P, v, ML,
-                               icg == 1 ? 1 : 0);
         }
         RS_TRY(hpsi_apply<T>(P, HP, nb, kKin | kLoc | kNl, act));
         {

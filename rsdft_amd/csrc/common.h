@@ -140,6 +140,8 @@ struct CGScalars {
     double *E, *E1, *rb, *res, *gkgk, *bk, *u1, *u2, *W, *rr0, *rr1, *pa, *pb;
     int *active, *accept, *pc_active;
     int *rejected;    // 1 when a step of this band was rejected in the last CG call
+    int *pc_ill;      // s-step replay unresolved: the step-by-step preconditioner runs
+    int *pc_ok;       // active and resolved by the s-step replay
     int nb = 0;       // bands of the last CG call
     int ensure(int MB);
 };
@@ -200,6 +202,8 @@ struct Context {
     PhaseEvents *pe = nullptr;   // events of the running entry point, null: untimed
     long hpsi_bands = 0;         // bands of unmasked H psi launches
     DevMem hpsi_count;           // device counter: bands of masked H psi launches
+    DevMem pc_fallback;          // device counters: band preconditionings by the recurrence
+                                 // where the s-step replay was unresolved (cumulative, last call)
     // BF16 split-GEMM scratch
     DevMem gemm_part;
 };
@@ -244,19 +248,19 @@ int copy_bands(const T *in, T *out, int nb);
 template <class T>
 int nl_apply(T *out, int nb, const T *coef, const int *band_active = nullptr);
 
-// preconditioner (precond_hprsdft) for all active bands: in g, in/out pg.
+// preconditioner (precond_hprsdft) for all active bands by the reference's
+// step-by-step recurrence: in g, in/out pg.
 template <class T>
 int precond_batched(const T *g, T *pg, int nb, const int *active);
 
-// s-step form of the same preconditioner (precond_sstep.hip)
-template <class T>
-int precond_sstep(const T *g, T *pg, int nb, const int *active, double *part, double *dcoef);
-// its two passes: coefficients d[b][0..mloop) and pgg[b] = (Pg, g) (may be
-// null), then out = g + sum d_k v_k, or with fuse = 1 out = that + beta out
-// (beta == null: overwrite)
+// s-step form of the same preconditioner (precond_sstep.hip), two passes:
+// coefficients d[b][0..mloop) and pgg[b] = (Pg, g) (may be null), with
+// ill[b] = 1 (ill may be null) where the coefficient replay cannot resolve
+// the recurrence (nearly dependent Krylov vectors); then out = g + sum d_k
+// v_k, or with fuse = 1 out = that + beta out (beta == null: overwrite)
 template <class T>
 int precond_sstep_coeffs(const T *g, int nb, const int *active, double *part, double *dcoef,
-                         double *pgg);
+                         double *pgg, int *ill);
 template <class T>
 int precond_sstep_apply(const T *g, T *out, int nb, const int *active, const double *dcoef,
                         int fuse, const double *beta);
@@ -298,6 +302,24 @@ template <class T>
 int density_dev(const T *x, int nb, const double *occ_dev, double *rho_dev);
 template <class T>
 int force_dev(const T *x, int nb, const double *occ_dev, double *force_dev_out);
+
+// SCF iteration (scf.hip): Hartree potential and E_H of a device density,
+// LDA V_xc and (E_xc, E_x, E_c), Methfessel-Paxton occupations (host), the
+// SCF state (input density, Vion, mixing history) and one iteration on the
+// device-resident psi; info = Etot, Ehwf, Eeig, Eloc, Eion, E_H, E_xc,
+// Efermi, sqerr_rho, sqerr_v
+int hartree_dev(const double *rho, double *vh, double *eh);
+int xc_dev(const double *rho, double *vxc, double *e3);
+int calc_fermi_host(const double *esp, int nb, double nel, double ekbt, int kinteg, double *occ,
+                    double *efermi);
+int scf_init(const double *vion, const double *rho, double nel, int mmix, double beta, double ekbt,
+             int kinteg);
+bool scf_ready();
+template <class T>
+int scf_iterate_dev(int nb, double *esp_dev, double *res_dev, double *esp, double *occ,
+                    double *info);
+int scf_fields(double *rho, double *vloc, double *vh, double *vxc);
+int scf_release();
 
 // element-wise helpers
 template <class T>

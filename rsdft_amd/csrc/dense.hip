@@ -104,7 +104,10 @@ __global__ void k_check_info(const int *info, int *flag)
 //         E_ij = (B_ij + l_j R_ij)/(l_j - l_i) between clusters,
 //         E_ij = R_ij / 2 inside a cluster and on the diagonal,
 //         X <- X (I + E),
-// and stops when max |B_ij| (i != j) <= 1e-12 max |H| and max |R_ij| <= 1e-12.
+// and stops when max |B_ij| (i != j) <= 1e-12 max |H| and max |R_ij| <= 1e-12, or
+// when both are below 1e-10 and stopped shrinking (the floor eps / gap of a
+// close pair outside every cluster); the last X is then orthonormalised by
+// X <- X (I + R/2), which leaves |X^T X - I| ~ |R|^2.
 // Every step is an n x n FP64 GEMM or a single small kernel, instead of the
 // ~1,500 small kernels of the tridiagonal-reduction eigensolver.  A cluster
 // larger than kClusterStart at the start (kClusterMax later), too many strong
@@ -127,6 +130,13 @@ __global__ void k_set_identity(double *A, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (i < n) A[(long)j * n + i] = i == j ? 1.0 : 0.0;
+}
+
+// F = I + R / 2
+__global__ void k_rs_half_step(const double *__restrict__ R, int n, double *__restrict__ F)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i < n) F[(long)j * n + i] = (i == j ? 1.0 : 0.0) + 0.5 * R[(long)j * n + i];
 }
 
 // st[1] = max |offdiag B|, st[2] = max |R|, st[3] = max |H| (when hnorm); R may
@@ -509,6 +519,7 @@ static int refine_eigh(const double *H, int n, double *V, double *D, int *ok)
     const double *Bc = H, *Rc = nullptr;   // B and R of the current X (X = I)
     double hs[8];
     bool converged = false;
+    double prev_off = -1.0;
     for (int it = 0; it < kRefineIters; ++it) {
         // 1-2: exact rotation inside every cluster
         hipLaunchKernelGGL(k_rs_clear_pairs, dim3(1), dim3(64), 0, c.stream, pairs);
@@ -542,11 +553,23 @@ static int refine_eigh(const double *H, int n, double *V, double *D, int *ok)
         if (hs[5] != 0.0 || hs[4] != 0.0) return kOK;   // cluster too large / Jacobi stalled
         // rounding level: the next Gram-Schmidt restores orthogonality below 1e-12 anyway
         if (hs[1] <= 1e-12 * hs[3] && hs[2] <= 1e-12) { converged = true; break; }
+        // a pair of gap d keeps |R| near eps / d (each step's correction divides
+        // rounding by d): below 1e-10 and no longer shrinking, the iteration
+        // has reached that floor
+        if (prev_off >= 0.0 && hs[1] > 0.25 * prev_off && hs[1] <= 1e-10 * hs[3] &&
+            hs[2] <= 1e-10) { converged = true; break; }
         if (it >= 2 && hs[1] > 1e-5 * hs[3]) return kOK;   // not converging quadratically
+        prev_off = hs[1];
     }
     if (!converged) return kOK;
     // eigenvalues: diagonal of the final B
     hipLaunchKernelGGL(k_rs_update, g2, b2, 0, c.stream, B, R, n, clu, F, lam);
+    // symmetric (Loewdin) orthonormalisation to first order, X <- X (I + R/2)
+    // with R = I - X^T X: the orthogonality defect goes from |R| to |R|^2
+    hipLaunchKernelGGL(k_rs_half_step, g2, b2, 0, c.stream, R, n, F);
+    RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_none, n, n, n,
+                          &one, X, n, F, n, &zero, Xn, n));
+    std::swap(X, Xn);
     hipLaunchKernelGGL(k_rs_sort, dim3(n), dim3(256), 0, c.stream, X, lam, n, V, D);
     RS_HIP(hipGetLastError());
     *ok = 1;

@@ -13,7 +13,8 @@
 // tridiagonal matrix T on coefficients and every inner product is c^T G c'
 // with G_jk = (v_j, v_k).  Pass 1 streams g once and accumulates the
 // (m+1)x(m+1) Gram matrix per band; one thread per band then replays the
-// reference's steps (same a_i, b_i, same early exit rr < 1e-24); pass 2
+// reference's steps (same a_i, b_i, same early exit rr < 1e-24; it stops
+// where r or p has vanished below the rounding of the Gram forms); pass 2
 // streams g once more and writes Pg = g + sum_k d_k v_k.  Two reads and one
 // write of psi-sized data replace ~30 passes of the step-by-step form.
 //
@@ -319,10 +320,17 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
 // (r0,r0) < 1e-24 (the reference's early return), which gives d = 0.
 // pgg[b] = (Pg, g) = (g,g) + sum_k d_k (v_k, g) from the same Gram matrix
 // (the inner product the CG step takes next, cg_module.f90:237-243).
+// Every inner product of the replay is a quadratic form c^T G c' whose
+// rounding is about eps |c|^T |G| |c'| (eps: the precision the basis was
+// computed in).  When the Krylov vectors of g are nearly dependent (a
+// gradient concentrated on few eigenmodes of A: smooth, or a plane wave)
+// the forms cancel to a small fraction `cond` of that bound and the replay
+// loses log10(1/cond) digits; ill[b] = 1 then (cond < relmin), and the
+// caller runs the step-by-step recurrence for that band instead.
 template <int M>
 __global__ void k_pc_sstep_scalars(const double *__restrict__ part, int nb, int tiles, double lmax,
                                    const int *__restrict__ active, double *__restrict__ dcoef,
-                                   double *__restrict__ pgg)
+                                   double *__restrict__ pgg, int *__restrict__ ill, double relmin)
 {
     constexpr int N = M + 1;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -347,6 +355,7 @@ __global__ void k_pc_sstep_scalars(const double *__restrict__ part, int nb, int 
     }
     double d[M];
     for (int k = 0; k < M; ++k) d[k] = 0.0;
+    double cond = 1.0;
     double rr1 = Gm[0][0];
     if (!(rr1 < 1.0e-24)) {
         double rc[N], pc[N], qc[N];
@@ -366,17 +375,28 @@ __global__ void k_pc_sstep_scalars(const double *__restrict__ part, int nb, int 
                     if (k + 1 < N) qc[k + 1] += qh * pc[k];
                 }
             }
-            double pAp = 0.0;
+            double pAp = 0.0, pAp_abs = 0.0;
             for (int j = 0; j < N; ++j)
-                for (int k = 0; k < N; ++k) pAp += pc[j] * Gm[j][k] * qc[k];
+                for (int k = 0; k < N; ++k) {
+                    const double t = pc[j] * Gm[j][k] * qc[k];
+                    pAp += t;
+                    pAp_abs += fabs(t);
+                }
+            cond = fmin(cond, fabs(pAp) / pAp_abs);
             const double rr0 = rr1;
             const double a = rr0 / pAp;
             for (int k = 0; k < M; ++k) d[k] += a * pc[k];
             if (il == M) break;
             for (int k = 0; k < N; ++k) rc[k] -= a * qc[k];
             rr1 = 0.0;
+            double rr_abs = 0.0;
             for (int j = 0; j < N; ++j)
-                for (int k = 0; k < N; ++k) rr1 += rc[j] * Gm[j][k] * rc[k];
+                for (int k = 0; k < N; ++k) {
+                    const double t = rc[j] * Gm[j][k] * rc[k];
+                    rr1 += t;
+                    rr_abs += fabs(t);
+                }
+            cond = fmin(cond, fabs(rr1) / rr_abs);
             const double bb = rr1 / rr0;
             for (int k = 0; k < N; ++k) pc[k] = rc[k] + bb * pc[k];
         }
@@ -387,6 +407,7 @@ __global__ void k_pc_sstep_scalars(const double *__restrict__ part, int nb, int 
         pg += d[k] * Gg[k + 1];
     }
     if (pgg) pgg[b] = pg;
+    if (ill) ill[b] = cond < relmin ? 1 : 0;
 }
 
 struct SstepGeom {
@@ -453,7 +474,7 @@ static SstepConst sstep_const(const GridDesc &gd)
 // pass 1: Gram of g, v_0 .. v_M, then the coefficients d and (Pg, g)
 template <class T, int M>
 static int sstep_coeffs(const T *g, int nb, const int *active, double *part, double *dcoef,
-                        double *pgg)
+                        double *pgg, int *ill)
 {
     Context &c = ctx();
     const GridDesc &gd = c.g;
@@ -470,8 +491,12 @@ static int sstep_coeffs(const T *g, int nb, const int *active, double *part, dou
                        gd.N1 / PX, k.c0, k.c1, k.c2, k.c3, k.s1, nb, active,
                        (const double *)nullptr, part, G.tiles, (const double *)nullptr);
     RS_HIP(hipGetLastError());
+    // the replay keeps 8 digits of (Pg, g) and of Pg's coefficients in FP64; an
+    // FP32 basis carries rounding ~6e-8 of each vector, so 1e-5 keeps 2 digits
+    // of a direction that FP32 storage rounds anyway
+    const double relmin = sizeof(T) == 8 ? 1.0e-8 : 1.0e-5;
     hipLaunchKernelGGL((k_pc_sstep_scalars<M>), dim3((nb + 127) / 128), dim3(128), 0, c.stream,
-                       part, nb, G.tiles, k.lmax, active, dcoef, pgg);
+                       part, nb, G.tiles, k.lmax, active, dcoef, pgg, ill, relmin);
     RS_HIP(hipGetLastError());
     return kOK;
 }
@@ -521,21 +546,14 @@ size_t precond_sstep_part_elems(int nb)
 }
 
 template <class T>
-int precond_sstep(const T *g, T *pg, int nb, const int *active, double *part, double *dcoef)
-{
-    RS_TRY(precond_sstep_coeffs<T>(g, nb, active, part, dcoef, nullptr));
-    return precond_sstep_apply<T>(g, pg, nb, active, dcoef, 0, nullptr);
-}
-
-template <class T>
 int precond_sstep_coeffs(const T *g, int nb, const int *active, double *part, double *dcoef,
-                         double *pgg)
+                         double *pgg, int *ill)
 {
     switch (ctx().mloop) {
-    case 1: return sstep_coeffs<T, 1>(g, nb, active, part, dcoef, pgg);
-    case 2: return sstep_coeffs<T, 2>(g, nb, active, part, dcoef, pgg);
-    case 3: return sstep_coeffs<T, 3>(g, nb, active, part, dcoef, pgg);
-    case 4: return sstep_coeffs<T, 4>(g, nb, active, part, dcoef, pgg);
+    case 1: return sstep_coeffs<T, 1>(g, nb, active, part, dcoef, pgg, ill);
+    case 2: return sstep_coeffs<T, 2>(g, nb, active, part, dcoef, pgg, ill);
+    case 3: return sstep_coeffs<T, 3>(g, nb, active, part, dcoef, pgg, ill);
+    case 4: return sstep_coeffs<T, 4>(g, nb, active, part, dcoef, pgg, ill);
     default: return set_error(kErrArg, "s-step preconditioner built for mloop 1..4");
     }
 }
@@ -553,12 +571,10 @@ int precond_sstep_apply(const T *g, T *out, int nb, const int *active, const dou
     }
 }
 
-template int precond_sstep<double>(const double *, double *, int, const int *, double *, double *);
-template int precond_sstep<float>(const float *, float *, int, const int *, double *, double *);
 template int precond_sstep_coeffs<double>(const double *, int, const int *, double *, double *,
-                                          double *);
+                                          double *, int *);
 template int precond_sstep_coeffs<float>(const float *, int, const int *, double *, double *,
-                                         double *);
+                                         double *, int *);
 template int precond_sstep_apply<double>(const double *, double *, int, const int *,
                                          const double *, int, const double *);
 template int precond_sstep_apply<float>(const float *, float *, int, const int *, const double *,

@@ -39,6 +39,8 @@ EXPORTS = (
     "rsdft_gpu_wf_sync", "rsdft_gpu_wf_detach", "rsdft_gpu_conjugate_gradient",
     "rsdft_gpu_gram_schmidt", "rsdft_gpu_subspace_diag", "rsdft_gpu_sweep",
     "rsdft_gpu_calc_density", "rsdft_gpu_force_ps_nloc2", "rsdft_gpu_band_energy_grad",
+    "rsdft_gpu_calc_hartree", "rsdft_gpu_calc_xc", "rsdft_gpu_calc_fermi", "rsdft_gpu_scf_init",
+    "rsdft_gpu_scf_iterate", "rsdft_gpu_scf_get_fields",
     "rsdft_gpu_test_hpsi", "rsdft_gpu_test_gemm", "rsdft_gpu_test_eigh", "rsdft_gpu_get_timers", "rsdft_gpu_reset_timers",
     "rsdft_gpu_finalize", "rsdft_gpu_last_error",
     "rsdft_gpu_write_wf", "rsdft_gpu_read_wf", "rsdft_gpu_write_vrho", "rsdft_gpu_read_vrho",
@@ -86,6 +88,12 @@ def load_library(path: str = LIB_PATH):
         "rsdft_gpu_calc_density": [P, I, I, P, P],
         "rsdft_gpu_force_ps_nloc2": [P, I, I, P, P],
         "rsdft_gpu_band_energy_grad": [P, I, I, P, P, P, P],
+        "rsdft_gpu_calc_hartree": [P, P, P],
+        "rsdft_gpu_calc_xc": [P, P, P],
+        "rsdft_gpu_calc_fermi": [P, I, D, D, I, P, P],
+        "rsdft_gpu_scf_init": [P, P, D, I, D, D, I],
+        "rsdft_gpu_scf_iterate": [P, I, I, P, P, P, P],
+        "rsdft_gpu_scf_get_fields": [P, P, P, P],
         "rsdft_gpu_test_eigh": [P, I, I, P, P, P],
         "rsdft_gpu_test_hpsi": [I, P],
         "rsdft_gpu_test_gemm": [I, P],
@@ -126,6 +134,18 @@ def _check(lib, name, r):
     if r != 0:
         msg = lib.rsdft_gpu_last_error().decode(errors="replace")
         raise RsdftGpuError(f"{name} -> {r}: {msg}")
+
+
+def calc_fermi(esp, nelectron, ekbt=1e-5, kinteg=5):
+    """calc_fermi with Methfessel-Paxton smearing (fermi_mp_module.f90:96-287):
+    returns (occ, efermi)."""
+    lib = load_library()
+    e = _f64(esp)
+    occ = np.zeros(e.size)
+    ef = np.zeros(1)
+    _check(lib, "calc_fermi", lib.rsdft_gpu_calc_fermi(_p(e), e.size, float(nelectron),
+                                                     float(ekbt), int(kinteg), _p(occ), _p(ef)))
+    return occ, float(ef[0])
 
 
 def write_wf(path, N, unk, occ, OC=1, MBwr=(1, 0), aa=None, bb=None):
@@ -338,6 +358,46 @@ class RsdftGpu:
         self._call("rsdft_gpu_calc_density", _p(x), self.ML, x.shape[0], _p(occ), _p(rho))
         return rho
 
+    # -- SCF pieces (scf_module.f90:146-366) ---------------------------------------
+    def calc_hartree(self, rho):
+        """(Vh, E_H) of the density rho (hartree_sol_module.f90:19-122)."""
+        r = _f64(rho)
+        vh = np.zeros(self.ML)
+        eh = np.zeros(1)
+        self._call("rsdft_gpu_calc_hartree", _p(r), _p(vh), _p(eh))
+        return vh, float(eh[0])
+
+    def calc_xc(self, rho):
+        """(Vxc, Exc, Ex, Ec) of the PZ81 LDA (xc_ldapz81_module.f90:17-75)."""
+        r = _f64(rho)
+        v = np.zeros(self.ML)
+        e = np.zeros(3)
+        self._call("rsdft_gpu_calc_xc", _p(r), _p(v), _p(e))
+        return v, float(e[0]), float(e[1]), float(e[2])
+
+    def scf_init(self, vion, rho, nelectron, mmix=4, beta=1.0, ekbt=1e-5, kinteg=5):
+        """init_scf + init_mixing for imix = 10; Vloc = Vion + Vh[rho] + Vxc[rho]."""
+        self._scf_in = (_f64(vion), _f64(rho))
+        self._call("rsdft_gpu_scf_init", _p(self._scf_in[0]), _p(self._scf_in[1]),
+                   float(nelectron), int(mmix), float(beta), float(ekbt), int(kinteg))
+
+    def scf_iterate(self, unk):
+        """One scf iteration: returns (psi, esp, occ, res, info dict)."""
+        x = self._wf(unk)
+        MB = x.shape[0]
+        esp, occ, res, info = np.zeros(MB), np.zeros(MB), np.zeros(MB), np.zeros(10)
+        self._call("rsdft_gpu_scf_iterate", _p(x), self.ML, MB, _p(esp), _p(occ), _p(res),
+                   _p(info))
+        keys = ("etot", "ehwf", "eeig", "eloc", "eion", "eh", "exc", "efermi", "sqerr_rho",
+                "sqerr_v")
+        return x, esp, occ, res, dict(zip(keys, (float(v) for v in info)))
+
+    def scf_fields(self):
+        """(rho, Vloc, Vh, Vxc) after the last SCF step."""
+        f = [np.zeros(self.ML) for _ in range(4)]
+        self._call("rsdft_gpu_scf_get_fields", *(_p(a) for a in f))
+        return tuple(f)
+
     def calc_force_ps_nloc2(self, unk, occ):
         x = self._wf(unk)
         occ = _f64(occ)
@@ -384,11 +444,11 @@ class RsdftGpu:
         return t
 
     def timers(self):
-        t = np.zeros(9)
-        self._call("rsdft_gpu_get_timers", _p(t), 9)
+        t = np.zeros(10)
+        self._call("rsdft_gpu_get_timers", _p(t), 10)
         return {"hamiltonian": t[0], "precond": t[1], "cg": t[2], "gram_schmidt": t[3],
                 "subspace_diag": t[4], "sweep": t[5], "hpsi_bands": t[6], "eig_refined": t[7],
-                "cg_rejected_bands": t[8]}
+                "cg_rejected_bands": t[8], "pc_recurrence_bands": t[9]}
 
     def reset_timers(self):
         self._call("rsdft_gpu_reset_timers")

