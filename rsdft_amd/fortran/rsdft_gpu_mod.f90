@@ -145,6 +145,51 @@ module rsdft_gpu_capi
       real(c_double), intent(inout) :: e, dedv(*), dedr(*)
       integer(c_int), value :: ml0, mb
     end function
+    ! SCF iteration (scf_module.f90:146-366)
+    integer(c_int) function rsdft_gpu_calc_hartree(rho, vh, e_h) bind(C, name="rsdft_gpu_calc_hartree")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: rho(*)
+      real(c_double), intent(out) :: vh(*), e_h
+    end function
+    integer(c_int) function rsdft_gpu_calc_xc(rho, vxc, e_xc) bind(C, name="rsdft_gpu_calc_xc")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: rho(*)
+      real(c_double), intent(out) :: vxc(*), e_xc(3)
+    end function
+    integer(c_int) function rsdft_gpu_calc_fermi(esp, nb, nelectron, ekbt, kinteg, occ, efermi) &
+         bind(C, name="rsdft_gpu_calc_fermi")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: esp(*)
+      integer(c_int), value :: nb, kinteg
+      real(c_double), value :: nelectron, ekbt
+      real(c_double), intent(out) :: occ(*), efermi
+    end function
+    integer(c_int) function rsdft_gpu_scf_init(vion, rho, nelectron, mmix, beta, ekbt, kinteg) &
+         bind(C, name="rsdft_gpu_scf_init")
+      import :: c_int, c_double
+      real(c_double), intent(in) :: vion(*), rho(*)
+      real(c_double), value :: nelectron, beta, ekbt
+      integer(c_int), value :: mmix, kinteg
+    end function
+    integer(c_int) function rsdft_gpu_scf_iterate(unk, ml0, mb, esp, occ, res, info) &
+         bind(C, name="rsdft_gpu_scf_iterate")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: unk(*)
+      integer(c_int), value :: ml0, mb
+      real(c_double), intent(out) :: esp(*), occ(*), res(*), info(10)
+    end function
+    integer(c_int) function rsdft_gpu_scf_get_fields(rho, vloc, vh, vxc) &
+         bind(C, name="rsdft_gpu_scf_get_fields")
+      import :: c_int, c_double
+      real(c_double), intent(out) :: rho(*), vloc(*), vh(*), vxc(*)
+    end function
+    ! allocation-free hamiltonian on device arrays (no reference counterpart)
+    integer(c_int) function rsdft_gpu_hamiltonian_noalloc(k, s, d_tpsi, d_htpsi, n1, n2, ib1, ib2) &
+         bind(C, name="rsdft_gpu_hamiltonian_noalloc")
+      import :: c_int, c_ptr
+      integer(c_int), value :: k, s, n1, n2, ib1, ib2
+      type(c_ptr), value :: d_tpsi, d_htpsi
+    end function
     integer(c_int) function rsdft_gpu_get_timers(t, n) bind(C, name="rsdft_gpu_get_timers")
       import :: c_int, c_double
       real(c_double), intent(inout) :: t(*)
