@@ -177,6 +177,68 @@ def pmc_summary(fetch_dir, write_dir, out_path):
     print(json.dumps(res, indent=1))
 
 
+# MI355X: 256 CUs in 8 XCDs, 4 SIMDs per CU; GRBM_GUI_ACTIVE counts the
+# cycles of every XCD, so one dispatch's duration in cycles is its sum / 8
+MI355X_CUS, MI355X_XCDS, SIMDS_PER_CU = 256, 8, 4
+SQ_PASSES = (("SQ_WAVES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES", "GRBM_GUI_ACTIVE"),
+             ("SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT", "SQ_BUSY_CYCLES", "SQ_INSTS_SALU"))
+
+
+def sq_summary(a_dir, b_dir, out_path):
+    """Per-dispatch SQ counters of the sweep's kernels from the two counter
+    runs of --sweep-probe, as a markdown table: VALU busy = 4 cycles per wave64
+    VALU instruction over the SIMD cycles of the dispatch; LDS conflict share =
+    bank-conflict cycles per CU over the dispatch's cycles."""
+    import csv
+    import glob
+    vals = {}
+    for d in (a_dir, b_dir):
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                vals.setdefault(r["Kernel_Name"], {}).setdefault(r["Counter_Name"], []).append(
+                    float(r["Counter_Value"]))
+    rows = []
+    for k, v in vals.items():
+        if not all(c in v for p in SQ_PASSES for c in p):
+            continue
+        m = {c: float(np.mean(x)) for c, x in v.items()}
+        cyc = m["GRBM_GUI_ACTIVE"] / MI355X_XCDS
+        if cyc <= 0:
+            continue
+        rows.append((cyc * len(v["GRBM_GUI_ACTIVE"]), k, len(v["GRBM_GUI_ACTIVE"]), cyc, m))
+    rows.sort(reverse=True)
+    lines = ["| kernel | dispatches | cycles | waves | VALU instr | VALU busy | LDS instr "
+             "| LDS bank-conflict cycles | conflict share | SALU instr |",
+             "|---|---|---|---|---|---|---|---|---|---|"]
+    for _, k, n, cyc, m in rows[:16]:
+        name = k.replace("void ", "").split("(")[0]
+        if len(name) > 70:
+            name = name[:70] + "…"
+        busy = 4 * m["SQ_ACTIVE_INST_VALU"] / (MI355X_CUS * SIMDS_PER_CU * cyc)
+        share = m["SQ_LDS_BANK_CONFLICT"] / MI355X_CUS / cyc
+        lines.append(f"| `{name}` | {n} | {cyc / 1e6:.2f}M | {m['SQ_WAVES']:.0f} "
+                     f"| {m['SQ_INSTS_VALU'] / 1e6:.1f}M | {100 * busy:.0f} % "
+                     f"| {m['SQ_INSTS_LDS'] / 1e6:.1f}M | {m['SQ_LDS_BANK_CONFLICT'] / 1e6:.2f}M "
+                     f"| {100 * share:.1f} % | {m['SQ_INSTS_SALU'] / 1e6:.1f}M |")
+    cmds = [f"cd /tmp && export TMPDIR=/tmp && timeout -s KILL 120 rocprofv3 --pmc {' '.join(p)} "
+            f"-d <repo>/gpurun_out/pmc_sq_{t} -o run -f csv -- python <repo>/bench.py --sweep-probe"
+            for t, p in zip("ab", SQ_PASSES)]
+    cmds.append("python bench.py --sq-summary gpurun_out/pmc_sq_a gpurun_out/pmc_sq_b")
+    text = ("# SQ counters: Si216 mixed-precision sweep kernels (per dispatch)\n\n"
+            "Commands (from the repository root on one MI355X; `<repo>` is that root):\n\n```\n"
+            + "\n".join(cmds) + "\n```\n\n"
+            "`--sweep-probe` runs one warm-up and two timed mixed-precision sweeps of Si216 "
+            "(72^3 grid, 480 bands).  Cycles: GRBM_GUI_ACTIVE / 8 XCDs.  VALU busy: "
+            "4 x SQ_ACTIVE_INST_VALU / (256 CUs x 4 SIMDs x cycles).  Conflict share: "
+            "SQ_LDS_BANK_CONFLICT / 256 CUs / cycles.  Sorted by total cycles.  rocprofv3 "
+            "may end with status 139 in the process's exit handlers after the program's "
+            "last line; the counter files are complete then.\n\n"
+            + "\n".join(lines) + "\n")
+    with open(out_path, "w") as f:
+        f.write(text)
+    print(text)
+
+
 def traffic_probe():
     """A few FP32 all-band applications of every hamiltonian piece and the copy
     (the counter runs profile this)."""
@@ -188,6 +250,23 @@ def traffic_probe():
     dev.attach(x)
     t = dev.test_hpsi(3)
     log(f"probe hpsi ms: {t}")
+    dev.finalize()
+
+
+def sweep_probe():
+    """Two mixed-precision Si216 sweeps after one warm-up sweep (the counter
+    runs of the sweep's kernels profile this)."""
+    from rsdft_amd.system import make_config
+    from rsdft_amd.gpu import RsdftGpu, PREC_MIXED_FP32
+    sysm = make_config("si216")
+    x = sysm.psi0()
+    dev = RsdftGpu(sysm, precision=PREC_MIXED_FP32)
+    dev.attach(x)
+    dev.gram_schmidt(x)
+    dev.sweep(x, 1)
+    dev.reset_timers()
+    dev.sweep(x, 2)
+    log(f"probe sweeps: {dev.timers()}")
     dev.finalize()
 
 
@@ -237,13 +316,21 @@ def main():
     ap.add_argument("--pc-mode", type=int, choices=[0, 1], default=1,
                     help="preconditioner: 1 s-step (default), 0 step-by-step recurrence")
     ap.add_argument("--traffic-probe", action="store_true")
+    ap.add_argument("--sweep-probe", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--sq-summary", nargs=2, metavar=("PASS_A_DIR", "PASS_B_DIR"))
     args = ap.parse_args()
+    if args.sq_summary:
+        sq_summary(*args.sq_summary, os.path.join(ROOT, "profiles", "sweep_si216_sq_counters.md"))
+        return
     if args.pmc_summary:
         pmc_summary(args.pmc_summary[0], args.pmc_summary[1], TRAFFIC_FILE)
         return
     if args.traffic_probe:
         traffic_probe()
+        return
+    if args.sweep_probe:
+        sweep_probe()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "

@@ -464,7 +464,11 @@ StencilGeom choose_hpsi_geom(int N1, int N2, int N3, int md, int px, int pad, in
         if (N2 % ty) continue;
         const int thr = tpr * ty;
         if (thr > max_threads) break;
-        const size_t lds = 2 * (size_t)(ty + 2 * md) * (N1 + 2 * pad) * elem_bytes;
+        // LDS rows: N1 + 2*pad points for the strided FP64 kernel, the padded
+        // chunk pitch for the chunked FP32 kernel
+        const int row = elem_bytes == 8 ? N1 + 2 * pad
+                                        : hpsi2_pitch(tpr, pad / px, px * elem_bytes) * px;
+        const size_t lds = 2 * (size_t)(ty + 2 * md) * row * elem_bytes;
         if (lds > 64 * 1024) continue;
         const int waves = (thr + 63) / 64;
         const double eff = (double)thr / (64.0 * waves);

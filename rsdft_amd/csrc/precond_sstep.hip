@@ -49,6 +49,12 @@ __device__ __forceinline__ long xcd_map(long L, long total) {
 // index in every unrolled step: no register moves between planes.
 constexpr int kSstepPF = 2;
 constexpr int kRing = 8;
+// LDS row pitch (points): the row [pad, x = -1, x = 0 .. N1-1, x = N1, pad]
+// takes N1 + 4; the pitch is rounded up to N1 modulo one LDS pass
+__host__ __device__ constexpr int sstep_pitch(int N1, int elem)
+{
+    return N1 + (4 * elem + kLdsPassBytes - 1) / kLdsPassBytes * kLdsPassBytes / elem;
+}
 __host__ __device__ constexpr int rneed(int l, int NLEV) {
     return l == NLEV ? 1
                      : ((NLEV - l + 1) > 3 ? (NLEV - l + 1) : 3) + (l == 0 ? kSstepPF : 0);
@@ -117,7 +123,7 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
     const int rows = ty + 2 * NLEV;
     // LDS row: [pad, x = -1, x = 0 .. N1-1, x = N1, pad]; a thread's PX
     // adjacent points start at an even index, so they move as one vector
-    const int W = N1 + 4;
+    const int W = sstep_pitch(N1, (int)sizeof(T));
     const int LS = rows * W;                 // one level plane in LDS
     const int tid = threadIdx.x;
     const int r = tid / tpr;                 // 0 .. rows-1
@@ -428,7 +434,7 @@ static SstepGeom sstep_geom(const GridDesc &g, int nlev, int nlds, int px, int m
         if (g.N2 % ty) continue;
         const int thr = tpr * (ty + 2 * nlev);
         if (thr > maxthr) break;
-        const size_t lds = 2 * (size_t)nlds * (ty + 2 * nlev) * (g.N1 + 4) * elem;
+        const size_t lds = 2 * (size_t)nlds * (ty + 2 * nlev) * sstep_pitch(g.N1, (int)elem) * elem;
         if (lds > maxlds) break;
         G.ty = ty;
         G.threads = thr;

@@ -39,6 +39,14 @@ struct alignas(sizeof(T) * PX) Chunk {
     T v[PX];
 };
 
+// LDS row pitch of k_hpsi2 in chunks: the tpr chunks of a row and nc pad
+// chunks on each side, rounded up so that the pitch is tpr modulo one LDS pass
+__host__ __device__ constexpr int hpsi2_pitch(int tpr, int nc, int chunk_bytes)
+{
+    return tpr + (2 * nc * chunk_bytes + kLdsPassBytes - 1) / kLdsPassBytes * kLdsPassBytes /
+                     chunk_bytes;
+}
+
 // ---------------------------------------------------------------------------
 // Md-order hamiltonian stencil: out = [kin] -1/2 lap(in) + [loc] V*in (+ out if ACC)
 //
@@ -75,7 +83,7 @@ k_hpsi2(const T *__restrict__ in, T *__restrict__ out, long ML, int N1, int N2, 
     extern __shared__ unsigned char smem_raw[];
     C *L = reinterpret_cast<C *>(smem_raw);
     const int tpr = N1 / PX;
-    const int WC = tpr + 2 * NC;
+    const int WC = hpsi2_pitch(tpr, NC, (int)sizeof(C));
     const int LP = (ty + 2 * MD) * WC;
     const long total = (long)tiles * nbands;
     const long wk = xcd_work_index(blockIdx.x, total);
