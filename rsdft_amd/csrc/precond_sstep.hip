@@ -55,6 +55,16 @@ __host__ __device__ constexpr int sstep_pitch(int N1, int elem)
 {
     return N1 + (4 * elem + kLdsPassBytes - 1) / kLdsPassBytes * kLdsPassBytes / elem;
 }
+// Columns of the kernel's static reduction array (Gram entries in the Gram
+// pass, one otherwise), and its bytes, which the dynamic LDS budget leaves out
+__host__ __device__ constexpr int sstep_red_cols(int nlev, int mode)
+{
+    return mode == 0 ? nlev * (nlev + 1) / 2 + nlev : 1;
+}
+constexpr size_t sstep_static_lds(int nlev, int mode)
+{
+    return 16 * (size_t)sstep_red_cols(nlev, mode) * sizeof(double);
+}
 __host__ __device__ constexpr int rneed(int l, int NLEV) {
     return l == NLEV ? 1
                      : ((NLEV - l + 1) > 3 ? (NLEV - l + 1) : 3) + (l == 0 ? kSstepPF : 0);
@@ -104,12 +114,12 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
            int nb, const int *__restrict__ active, const double *__restrict__ dcoef,
            double *__restrict__ part, int tiles, const double *__restrict__ beta)
 {
-    constexpr int NG = NLEV * (NLEV + 1) / 2 + NLEV;  // Gram entries
+    constexpr int NG = sstep_red_cols(NLEV, 0);        // Gram entries
     constexpr int QTOT = roff(NLEV, NLEV) + rdepth(NLEV, NLEV);
     constexpr int PF = kSstepPF;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     T *Lbase = reinterpret_cast<T *>(smem_raw);
-    __shared__ double red[16][MODE == 0 ? NG : 1];
+    __shared__ double red[16][sstep_red_cols(NLEV, MODE)];
 
     const long total = (long)tiles * nb;
     const long w = xcd_map(blockIdx.x, total);
@@ -487,7 +497,7 @@ static int sstep_coeffs(const T *g, int nb, const int *active, double *part, dou
     constexpr int PX = 2;
     const SstepConst k = sstep_const(gd);
     const SstepGeom G = sstep_geom(gd, M + 1, M + 1, PX, sstep_max_threads<T, M + 1>(),
-                                   sizeof(T), c.max_lds);
+                                   sizeof(T), c.max_lds - sstep_static_lds(M + 1, 0));
     if (G.ty == 0) return set_error(kErrArg, "no s-step tiling");
     const size_t lds = G.lds;
     const long nblk = (long)G.tiles * nb;
@@ -516,7 +526,8 @@ static int sstep_apply(const T *g, T *out, int nb, const int *active, const doub
     const GridDesc &gd = c.g;
     constexpr int PX = 2;
     const SstepConst k = sstep_const(gd);
-    const SstepGeom G = sstep_geom(gd, M, M, PX, sstep_max_threads<T, M>(), sizeof(T), c.max_lds);
+    const SstepGeom G = sstep_geom(gd, M, M, PX, sstep_max_threads<T, M>(), sizeof(T),
+                                   c.max_lds - sstep_static_lds(M, 1));
     if (G.ty == 0) return set_error(kErrArg, "no s-step tiling");
     const size_t lds = G.lds;
     const long nblk = (long)G.tiles * nb;
@@ -541,10 +552,11 @@ size_t precond_sstep_part_elems(int nb)
     Context &c = ctx();
     size_t mx = 0;
     for (int m = 1; m <= 4; ++m) {
+        const size_t budget = c.max_lds - sstep_static_lds(m + 1, 0);
         const SstepGeom G = sstep_geom(c.g, m + 1, m + 1, 2, sstep_max_threads_rt(8, m + 1), 8,
-                                       c.max_lds);
+                                       budget);
         const SstepGeom G4 = sstep_geom(c.g, m + 1, m + 1, 2, sstep_max_threads_rt(4, m + 1), 4,
-                                        c.max_lds);
+                                        budget);
         const size_t t = (size_t)std::max(G.tiles, G4.tiles);
         mx = std::max(mx, t * (size_t)((m + 1) * (m + 2) / 2 + m + 1) * nb);
     }

@@ -1,0 +1,83 @@
+"""The reference's shipped Si216 example on the GPU: the real Si
+pseudopotential (examples/Si216/Si_psv.dat), 216 atoms at the positions of
+fort.970, a 48^3 grid of the 30.783 bohr cube, Md = 6, 648 bands, LDA (PZ81),
+10 sweeps at the initial potential then the SCF loop (fort.1: Nsweep 10,
+SWSCF 1) -- the set-up of rsdft_amd/pseudo.py, the FP64 sweeps and SCF of the
+HIP library -- against what the reference printed for the same run
+(tests/golden/si216_psv/results.json):
+
+* the first SCF iteration, whose input is the converged fixed-potential
+  state: total and Harris energies (a.log:403-405, 8 decimals) and the two
+  mixing errors (6 significant digits);
+* the converged SCF: the total energy of fort.99 to 1e-9 Ha, the occupied
+  eigenvalues to 1e-7 Ha (the reference's own residual estimate esp_err is
+  <= 1.7e-9 Ha for them) and every eigenvalue whose esp_err is below 1e-9.
+The path in between (mixing history, random start) differs from the
+reference's and is not compared.
+"""
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+from rsdft_amd import pseudo
+from rsdft_amd.gpu import RsdftGpu, PREC_FP64
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "si216_psv")
+
+
+def test_si216_psv_scf_matches_reference_run(gpu_ctx):
+    ref = json.load(open(os.path.join(GOLD, "results.json")))
+    at = json.load(open(os.path.join(GOLD, "atoms.json")))
+    ps = pseudo.read_psv(os.path.join(GOLD, "Si_psv.dat.gz"))
+    t0 = time.time()
+    P = pseudo.make_psv_system(ps, np.eye(3) * at["AX"], np.array(at["frac"]), at["Ngrid"],
+                               Md=at["Md"], name="si216_psv")
+    s = P.system
+    print(f"case: si216 psv 48^3, {s.nband} bands, {s.proj.nzlma} projectors; set-up "
+          f"{time.time() - t0:.1f} s, Ewald {P.eewald:.12f}", flush=True)
+    assert s.nband == ref["nband"]
+    dev = RsdftGpu(s, precision=PREC_FP64, with_derivatives=False)
+    try:
+        x = s.psi0()
+        dev.attach(x)
+        dev.gram_schmidt(x)
+        dev.scf_init(P.vion, P.rho0, P.nelectron)
+        t1 = time.time()
+        _, e, res = dev.sweep(x, 10)
+        print(f"  10 sweeps at the initial potential: {time.time() - t1:.2f} s, "
+              f"max occupied residual {np.max(res[:432]):.1e}", flush=True)
+        (t_ref, h_ref), (r_ref, v_ref) = ref["scf_trace"][0], ref["scf_sqerr_rho_v"][0]
+        prev = None
+        for it in range(40):
+            x, e, occ, res, info = dev.scf_iterate(x)
+            et, eh = info["etot"] + P.eewald, info["ehwf"] + P.eewald
+            print(f"  scf {it + 1}: Etot {et:.10f} Harris {eh:.10f} sqerr_rho "
+                  f"{info['sqerr_rho']:.4e} sqerr_v {info['sqerr_v']:.4e}", flush=True)
+            if it == 0:
+                assert abs(et - t_ref) < 1e-7 and abs(eh - h_ref) < 1e-7
+                assert abs(info["sqerr_rho"] / r_ref - 1) < 1e-4
+                assert abs(info["sqerr_v"] / v_ref - 1) < 1e-4
+            if prev is not None and abs(et - prev) < 1e-11 and abs(et - eh) < 1e-9:
+                break
+            prev = et
+        f = ref["final"]
+        print(f"  converged: Etot {et:.12f} vs {f['etot']:.12f} ({et - f['etot']:.1e})",
+              flush=True)
+        assert abs(et - f["etot"]) < 1e-9
+        esp_ref = np.array(f["esp"])
+        err = np.abs(np.array(f["esp_err"]))
+        d = np.abs(e - esp_ref)
+        nocc = int(np.sum(np.array(f["occ"]) > 0))
+        good = err < 1e-9
+        print(f"  occupied eigenvalues max diff {np.max(d[:nocc]):.1e}; {np.sum(good)} bands "
+              f"with esp_err < 1e-9: max diff {np.max(d[good]):.1e}", flush=True)
+        assert np.max(d[:nocc]) < 1e-7
+        assert np.max(d[good]) < 1e-7
+        assert np.allclose(occ[:nocc], 2.0) and np.allclose(occ[nocc:], 0.0, atol=1e-12)
+    finally:
+        dev.finalize()
