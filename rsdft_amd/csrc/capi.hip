@@ -354,6 +354,7 @@ int rsdft_gpu_set_projectors(int nzlma, int maxmjj, const int *MJJ, const int *J
     const long ML = c.g.ML;
     nl.nz = 0;
     nl.ng = 0;
+    nl.natom = 0;   // derivative tables belong to the previous projectors
     if (nzlma <= 0) return kOK;
     // per projector: (grid point, uV) sorted by point, repeated points merged
     // (a sphere larger than the cell lists a point once per image)

@@ -23,7 +23,7 @@ import numpy as np
 import pytest
 
 from rsdft_amd import pseudo
-from rsdft_amd.gpu import RsdftGpu, PREC_FP64
+from rsdft_amd.gpu import RsdftGpu, RsdftGpuError, PREC_FP64
 
 pytestmark = pytest.mark.gpu
 
@@ -79,5 +79,8 @@ def test_si216_psv_scf_matches_reference_run(gpu_ctx):
         assert np.max(d[:nocc]) < 1e-7
         assert np.max(d[good]) < 1e-7
         assert np.allclose(occ[:nocc], 2.0) and np.allclose(occ[nocc:], 0.0, atol=1e-12)
+        # no derivative tables for these projectors: the force entry refuses
+        with pytest.raises(RsdftGpuError, match="derivative tables"):
+            dev.calc_force_ps_nloc2(x, occ)
     finally:
         dev.finalize()
