@@ -281,6 +281,40 @@ RSDFT_API int rsdft_gpu_test_gemm(int nrep, double *times_ms);
 RSDFT_API int rsdft_gpu_get_timers(double *t, int n);
 RSDFT_API int rsdft_gpu_reset_timers(void);
 
+/* ---- complex wavefunctions at one k point (orthorhombic cell) ----------
+ * psi = exp(ik.r) u; the arrays below hold u as complex(8) (interleaved
+ * real/imaginary doubles), band-major (ML0, MB) as the reference's
+ * complex unk(n1:n2, MB).  FP64 throughout.
+ *
+ * The k point's kinetic terms and Bloch-phased projector tables: const_k2 =
+ * |k|^2/2 and coef_nabk(3, Md) = nab(m)/h_d * k_d (kinetic_module.f90:
+ * 118-149: zcoef_kin(+m) = coef_lap - i coef_nabk), uVk complex(maxmjj,
+ * nzlma) = uV exp(-i k.x) summed over the images of a point
+ * (prep_uvk_ps_nloc2, ps_nloc2_module.f90:1128-1190), MJJ / JJP / iuV as
+ * for rsdft_gpu_set_projectors.  The real local potential and the real FD
+ * Laplacian are the context's (rsdft_gpu_init, rsdft_gpu_set_vloc). */
+RSDFT_API int rsdft_gpu_set_kpoint(double const_k2, const double *coef_nabk, int nzlma,
+                                   int maxmjj, const int *MJJ, const int *JJP,
+                                   const double *uVk, const int *iuV);
+/* hamiltonian(k,s,tpsi,htpsi,n1,n2,ib1,ib2) for complex(8) tpsi/htpsi
+ * (hamiltonian_module.f90:20-72, complex branch: op_kinetic_sol with
+ * zcoef_kin, op_localpot, op_nonlocal with uVk).  k must be 1. */
+RSDFT_API int rsdft_gpu_zhamiltonian(int k, int s, const double *tpsi, double *htpsi, int n1,
+                                     int n2, int ib1, int ib2);
+/* conjugate_gradient_1, complex branch (cg_module.f90:97-430 with the zhegv
+ * 2x2 step of :299-318, precond_hprsdft cgpc_hprsdft_module.f90:126-263):
+ * unk complex(ML0, MB) in/out, esp and res out. */
+RSDFT_API int rsdft_gpu_zconjugate_gradient(double *unk, double *esp, double *res, int ML0,
+                                            int MB);
+/* gram_schmidt (gram_schmidt_t_module.f90:84-316) on complex unk. */
+RSDFT_API int rsdft_gpu_zgram_schmidt(double *unk, int ML0, int MB);
+/* subspace_diag (subspace_diag_la_module.f90:85-206, zheevd) on complex unk. */
+RSDFT_API int rsdft_gpu_zsubspace_diag(double *unk, double *esp, int ML0, int MB);
+/* nsweep iterations of conjugate_gradient -> gram_schmidt -> subspace_diag
+ * (sweep_module.f90:147-158) at the k point. */
+RSDFT_API int rsdft_gpu_zsweep(double *unk, double *esp, double *res, int ML0, int MB,
+                               int nsweep);
+
 /* Release every device resource. */
 RSDFT_API int rsdft_gpu_finalize(void);
 

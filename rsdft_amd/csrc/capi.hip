@@ -1129,6 +1129,7 @@ int rsdft_gpu_finalize(void)
                      &c.nl.amap_dev, &c.hpsi_count, &c.pc_fallback};
     for (DevMem *m : all) m->release();
     (void)scf_release();
+    kpoint_release();
     if (c.blas) rocblas_destroy_handle(c.blas);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.blas = nullptr;

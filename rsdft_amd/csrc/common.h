@@ -249,6 +249,11 @@ template <class T>
 int hpsi_stencil_only(const T *in, T *out, int nb);
 template <class T>
 int nl_stage(T *out, int nb, const T *coef);
+// kinetic + local stencil alone, overwriting out (nb rows)
+template <class T>
+int stencil_kinloc(const T *in, T *out, int nb);
+// device state of the complex k-point path
+void kpoint_release();
 template <class T>
 int copy_bands(const T *in, T *out, int nb);
 // out += V_NL in from the projection coefficients coef[lma][band]

@@ -739,6 +739,12 @@ int hpsi_stencil_only(const T *in, T *out, int nb)
 }
 
 template <class T>
+int stencil_kinloc(const T *in, T *out, int nb)
+{
+    return launch_stencil<T>(in, out, nb, kKin | kLoc, nullptr);
+}
+
+template <class T>
 int nl_stage(T *out, int nb, const T *coef)
 {
     if (fill_ready()) return nl_fill<T>(nb, coef, nullptr, out);
@@ -796,6 +802,7 @@ int pc_stencil(const T *in, T *Q, T *R, T *PP, T *PG, int nb, int mode, const in
 template int hpsi_apply<double>(const double *, double *, int, int, const int *, double *);
 template int hpsi_apply<float>(const float *, float *, int, int, const int *, float *);
 template int hpsi_stencil_only<double>(const double *, double *, int);
+template int stencil_kinloc<double>(const double *, double *, int);
 template int hpsi_stencil_only<float>(const float *, float *, int);
 template int nl_stage<double>(double *, int, const double *);
 template int nl_stage<float>(float *, int, const float *);

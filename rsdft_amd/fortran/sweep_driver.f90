@@ -4,7 +4,8 @@
 ! routine signatures.  Reads one stream-unformatted input file (grid, FD
 ! coefficients, Vloc, projector tables, psi), applies the hamiltonian once,
 ! orthonormalises, runs nsweep CG -> GS -> subspace sweeps, and writes
-! H psi(initial), esp, res and the final psi to the output file.
+! H psi(initial), esp, res, the final psi and H psi(initial) once more, from
+! the op_* wrappers entered by four OpenMP threads, to the output file.
 !   usage: sweep_driver <input> <output>
 !-----------------------------------------------------------------------
 program sweep_driver
@@ -13,7 +14,7 @@ program sweep_driver
   implicit none
   integer :: N(3), Md, nzlma, maxmjj, MB, nsweep, ML, it
   real(8) :: h(3), dV, coef_lap0
-  real(8), allocatable :: coef_lap(:,:), vloc(:), uVk(:,:), unk(:,:), hunk(:,:)
+  real(8), allocatable :: coef_lap(:,:), vloc(:), uVk(:,:), unk(:,:), hunk(:,:), hunk2(:,:)
   real(8), allocatable :: esp(:), res(:)
   integer, allocatable :: MJJ(:), JJP(:,:), iuV(:)
   character(len=1024) :: fin, fout
@@ -35,6 +36,15 @@ program sweep_driver
                        "set_projectors")
 
   call gpu_hamiltonian(1, 1, unk, hunk, 1, ML, 1, MB)
+  ! the same operator the way hamiltonian_module.f90:38-70 calls it: every
+  ! thread of an OpenMP region enters op_kinetic, op_localpot, op_nonlocal
+  allocate(hunk2(ML, MB))
+  hunk2 = 0.d0
+!$OMP parallel num_threads(4) default(shared)
+  call gpu_op_kinetic(1, unk, hunk2, 1, ML, 1, MB)
+  call gpu_op_localpot(1, ML, MB, unk, hunk2)
+  call gpu_op_nonlocal(1, 1, unk, hunk2, 1, ML, 1, MB)
+!$OMP end parallel
   call gpu_gram_schmidt(unk, ML, MB, 1, MB)
   esp = 0.d0
   res = 0.d0
@@ -45,7 +55,7 @@ program sweep_driver
   end do
 
   open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write(11) hunk, esp, res, unk
+  write(11) hunk, esp, res, unk, hunk2
   close(11)
   call rsdft_gpu_check(rsdft_gpu_finalize(), "finalize")
 end program sweep_driver

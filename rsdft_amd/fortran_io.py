@@ -27,5 +27,6 @@ def read_driver_output(path, ML, MB):
     hpsi = a[o:o + ML * MB].reshape(MB, ML); o += ML * MB
     esp = a[o:o + MB]; o += MB
     res = a[o:o + MB]; o += MB
-    psi = a[o:o + ML * MB].reshape(MB, ML)
-    return hpsi, esp, res, psi
+    psi = a[o:o + ML * MB].reshape(MB, ML); o += ML * MB
+    hpsi_omp = a[o:o + ML * MB].reshape(MB, ML)
+    return hpsi, esp, res, psi, hpsi_omp

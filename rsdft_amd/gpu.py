@@ -44,6 +44,8 @@ EXPORTS = (
     "rsdft_gpu_test_hpsi", "rsdft_gpu_test_gemm", "rsdft_gpu_test_eigh", "rsdft_gpu_get_timers", "rsdft_gpu_reset_timers",
     "rsdft_gpu_finalize", "rsdft_gpu_last_error",
     "rsdft_gpu_write_wf", "rsdft_gpu_read_wf", "rsdft_gpu_write_vrho", "rsdft_gpu_read_vrho",
+    "rsdft_gpu_set_kpoint", "rsdft_gpu_zhamiltonian", "rsdft_gpu_zconjugate_gradient",
+    "rsdft_gpu_zgram_schmidt", "rsdft_gpu_zsubspace_diag", "rsdft_gpu_zsweep",
 )
 
 
@@ -104,6 +106,12 @@ def load_library(path: str = LIB_PATH):
         "rsdft_gpu_read_wf": [ctypes.c_char_p, I, I, I, P, I, I, P, P],
         "rsdft_gpu_write_vrho": [ctypes.c_char_p, I, I, I, I, I, P, P, P, P, P, I, I, P, P, P, D],
         "rsdft_gpu_read_vrho": [ctypes.c_char_p, I, I, I, I, I, P, P, P, P, P],
+        "rsdft_gpu_set_kpoint": [D, P, I, I, P, P, P, P],
+        "rsdft_gpu_zhamiltonian": [I, I, P, P, I, I, I, I],
+        "rsdft_gpu_zconjugate_gradient": [P, P, P, I, I],
+        "rsdft_gpu_zgram_schmidt": [P, I, I],
+        "rsdft_gpu_zsubspace_diag": [P, P, I, I],
+        "rsdft_gpu_zsweep": [P, P, P, I, I, I],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -397,6 +405,57 @@ class RsdftGpu:
         f = [np.zeros(self.ML) for _ in range(4)]
         self._call("rsdft_gpu_scf_get_fields", *(_p(a) for a in f))
         return tuple(f)
+
+    # -- complex wavefunctions at one k point --------------------------------------
+    def set_kpoint(self, kvec, proj):
+        """The k point's kinetic terms and Bloch-phased projector tables (complex
+        ``proj.uVk``), e.g. from ``system.make_projectors(..., kvec=kvec)``."""
+        from .system import kpoint_kinetic
+        g = self.sys.grid
+        ck2, nabk = kpoint_kinetic(self.sys.Md, g.h, kvec)
+        self._kp_nabk = _f64(np.asarray(nabk).T.ravel())          # Fortran (3, Md)
+        u = np.ascontiguousarray(np.asarray(proj.uVk, dtype=np.complex128))
+        self._kp_tabs = (_i32(proj.MJJ), _i32(proj.JJP), u.view(np.float64), _i32(proj.iuV))
+        MJJ, JJP, uv, iuV = self._kp_tabs
+        self._call("rsdft_gpu_set_kpoint", float(ck2), _p(self._kp_nabk), proj.nzlma,
+                   proj.MAXMJJ, _p(MJJ), _p(JJP), _p(uv), _p(iuV))
+
+    @staticmethod
+    def _z(a):
+        return np.ascontiguousarray(np.atleast_2d(np.asarray(a, dtype=np.complex128))).copy()
+
+    def zhamiltonian(self, psi):
+        t = self._z(psi)
+        h = np.empty_like(t)
+        self._call("rsdft_gpu_zhamiltonian", 1, 1, _p(t.view(np.float64)), _p(h.view(np.float64)),
+                   1, self.ML, 1, t.shape[0])
+        return h
+
+    def zconjugate_gradient(self, psi):
+        x = self._z(psi)
+        esp, res = np.zeros(x.shape[0]), np.zeros(x.shape[0])
+        self._call("rsdft_gpu_zconjugate_gradient", _p(x.view(np.float64)), _p(esp), _p(res),
+                   self.ML, x.shape[0])
+        return x, esp, res
+
+    def zgram_schmidt(self, psi):
+        x = self._z(psi)
+        self._call("rsdft_gpu_zgram_schmidt", _p(x.view(np.float64)), self.ML, x.shape[0])
+        return x
+
+    def zsubspace_diag(self, psi):
+        x = self._z(psi)
+        esp = np.zeros(x.shape[0])
+        self._call("rsdft_gpu_zsubspace_diag", _p(x.view(np.float64)), _p(esp), self.ML,
+                   x.shape[0])
+        return x, esp
+
+    def zsweep(self, psi, nsweep: int = 1):
+        x = self._z(psi)
+        esp, res = np.zeros(x.shape[0]), np.zeros(x.shape[0])
+        self._call("rsdft_gpu_zsweep", _p(x.view(np.float64)), _p(esp), _p(res), self.ML,
+                   x.shape[0], int(nsweep))
+        return x, esp, res
 
     def calc_force_ps_nloc2(self, unk, occ):
         x = self._wf(unk)

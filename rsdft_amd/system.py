@@ -59,6 +59,7 @@ VLOC_GMAX2 = 9           # |n|^2 <= 9, n integer, n != 0
 STREAM_PSI = 1
 STREAM_VLOC = 2
 STREAM_D = 3
+STREAM_PSI_IM = 4
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 
@@ -133,6 +134,27 @@ def kinetic_coefficients(Md: int, h) -> tuple[float, np.ndarray]:
     coef_lap0 = lap[0] * (c[0] + c[1] + c[2])
     coef_lap = np.array([[lap[m] * c[d] for m in range(1, Md + 1)] for d in range(3)])
     return float(coef_lap0), coef_lap
+
+
+def fd_nab_closed_form(Md: int) -> np.ndarray:
+    """nab(1..Md) of the 2*Md-order central first derivative, closed form:
+    nab(j) = (-1)^(j+1) (Md!)^2 / (j (Md-j)! (Md+j)!) (the weights
+    get_coef_nabla_fd builds, fd_module.f90; nab(-j) = -nab(j))."""
+    f = math.factorial
+    return np.array([(-1) ** (j + 1) * f(Md) ** 2 / (j * f(Md - j) * f(Md + j))
+                     for j in range(1, Md + 1)])
+
+
+def kpoint_kinetic(Md: int, h, kvec) -> tuple[float, np.ndarray]:
+    """const_k2 and coef_nabk(3, Md) of one k point, orthorhombic cell
+    (kinetic_module.f90:118-149): coef_nabk(d, m) = nab(m)/h_d * k_d and
+    const_k2 = |k|^2/2, k Cartesian (bohr^-1).  The stencil then applies
+    zcoef(+m) = coef_lap - i coef_nabk to psi(x + m) and its conjugate to
+    psi(x - m): -1/2 (grad + ik)^2 on the periodic part of a Bloch function."""
+    nab = fd_nab_closed_form(Md)
+    k = np.asarray(kvec, dtype=np.float64)
+    coef_nabk = np.array([[nab[m] / h[d] * k[d] for m in range(Md)] for d in range(3)])
+    return float(0.5 * np.dot(k, k)), coef_nabk
 
 
 # ----------------------------------------------------------------------------
@@ -235,13 +257,19 @@ class Projectors:
 
 
 def make_projectors(grid: Grid, atom_ic: np.ndarray, atom_delta: np.ndarray,
-                    D: np.ndarray, sphere_delta: np.ndarray | None = None) -> Projectors:
+                    D: np.ndarray, sphere_delta: np.ndarray | None = None,
+                    kvec=None) -> Projectors:
     """Build uV/JJP/MJJ tables for atoms at ic*h + delta (ic integer grid index).
 
     ``sphere_delta`` (default: ``atom_delta``) is the atom offset used for the
     sphere membership test only; passing the unmoved offsets freezes every
     point set while the values follow the moved atom, which is the function
     whose R-derivative the nonlocal force is.
+
+    With ``kvec`` (Cartesian, bohr^-1) the values are Bloch-phased,
+    uVk = uV exp(-i k.x) with x the unfolded position of the point (the image
+    it came from), summed over the images of one grid point
+    (prep_uvk_ps_nloc2, ps_nloc2_module.f90:1128-1190): complex tables.
     """
     h = np.array(grid.h)
     N = np.array(grid.N)
@@ -263,22 +291,32 @@ def make_projectors(grid: Grid, atom_ic: np.ndarray, atom_delta: np.ndarray,
         r_in = np.sqrt(r2[keep])
         r_out = np.sqrt(r2[~keep])
         gap = min(gap, float(np.min(RC - r_in)), float(np.min(r_out - RC)))
+        if kvec is not None:
+            xu = (atom_ic[a][None, :] + offs[keep]) * h[None, :]
+            phase = np.exp(-1j * (xu @ np.asarray(kvec, dtype=np.float64)))
         for ch in range(4):
             val, dval = projector_values(xs, float(D[a, ch]), ch)
+            if kvec is not None:
+                val = val * phase
             order = np.argsort(idx, kind="stable")
             si, sv, sd = idx[order], val[order], dval[order]
             full.append((si + 1, sd))
             uniq, inv = np.unique(si, return_inverse=True)
-            summed = np.zeros(len(uniq))
+            summed = np.zeros(len(uniq), dtype=sv.dtype)
             np.add.at(summed, inv, sv)
             nonzero = np.zeros(len(uniq), dtype=bool)
             np.logical_or.at(nonzero, inv, sv != 0.0)
             lists.append((uniq[nonzero] + 1, summed[nonzero]))
+    if not lists:                       # no atoms: an empty operator
+        z = np.zeros(0, np.int32)
+        return Projectors(0, z, np.zeros((0, 1), np.int32),
+                          np.zeros((0, 1), np.complex128 if kvec is not None else np.float64),
+                          z, z, z, z, z, np.zeros((0, 1), np.int32), np.zeros((0, 1, 3)), 0.0)
     maxmjj = max(len(li[0]) for li in lists)
     mmjj = max(len(f[0]) for f in full)
     MJJ = np.zeros(nz, np.int32)
     JJP = np.zeros((nz, maxmjj), np.int32)
-    uVk = np.zeros((nz, maxmjj))
+    uVk = np.zeros((nz, maxmjj), dtype=np.complex128 if kvec is not None else np.float64)
     MJJ_MAP = np.zeros(nz, np.int32)
     JJ_MAP = np.zeros((nz, mmjj), np.int32)
     duVdR = np.zeros((nz, mmjj, 3))
@@ -331,6 +369,18 @@ class System:
         ML = self.grid.ML
         u = splitmix_uniform(self.seed, STREAM_PSI, nb * ML, offset=band0 * ML)
         return (2.0 * u - 1.0).reshape(nb, ML).astype(dtype, copy=False)
+
+    def psi0c(self, nbands: int | None = None) -> np.ndarray:
+        """Complex start for a k point: real part psi0(), imaginary part
+        uniform(-1, 1) from its own stream; shape (nbands, ML)."""
+        nb = self.nband if nbands is None else nbands
+        ML = self.grid.ML
+        v = splitmix_uniform(self.seed, STREAM_PSI_IM, nb * ML)
+        return self.psi0(0, nb) + 1j * (2.0 * v - 1.0).reshape(nb, ML)
+
+    def kpoint_projectors(self, kvec) -> Projectors:
+        """Bloch-phased tables of this system's projectors at k (complex uVk)."""
+        return make_projectors(self.grid, self.atom_ic, self.atom_delta, self.D, kvec=kvec)
 
 
 CONFIGS = {

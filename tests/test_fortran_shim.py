@@ -1,6 +1,8 @@
 """The Fortran side of the boundary: the ISO_C_BINDING module and a driver
 with the reference's call sequence compile and link against the library
-(CPU), and run end to end on the GPU against the oracle."""
+(CPU), and run end to end on the GPU against the oracle, including the op_*
+wrappers entered by every thread of an OpenMP parallel region (their master
+thread calls the library, the others wait at the barrier)."""
 import os
 import subprocess
 
@@ -32,9 +34,11 @@ def test_fortran_driver_matches_oracle(tmp_path, si8, oracle_si8):
     write_driver_input(fin, si8, psi, 2)
     r = subprocess.run([B.DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    hpsi, esp, res, x = read_driver_output(fout, si8.grid.ML, 8)
+    hpsi, esp, res, x, hpsi_omp = read_driver_output(fout, si8.grid.ML, 8)
     href = oracle_si8.hamiltonian(psi)
     assert np.max(np.abs(hpsi - href)) / np.max(np.abs(href)) < 1e-12
+    # op_kinetic + op_localpot + op_nonlocal entered by four OpenMP threads
+    assert np.max(np.abs(hpsi_omp - href)) / np.max(np.abs(href)) < 1e-12
     xo = oracle_si8.gram_schmidt(psi)
     for _ in range(2):
         xo, eo, ro = oracle_si8.sweep(xo)
