@@ -190,6 +190,52 @@ module rsdft_gpu_capi
       integer(c_int), value :: k, s, n1, n2, ib1, ib2
       type(c_ptr), value :: d_tpsi, d_htpsi
     end function
+    ! complex wavefunctions at one k point: complex(8) arrays as the
+    ! reference's complex unk(n1:n2, MB), zcoef_kin given as coef_nabk(3,Md)
+    ! and const_k2, Bloch-phased uVk(maxmjj, nzlma)
+    integer(c_int) function rsdft_gpu_set_kpoint(const_k2, coef_nabk, nzlma, maxmjj, mjj, jjp, &
+         uvk, iuv) bind(C, name="rsdft_gpu_set_kpoint")
+      import :: c_int, c_double, c_double_complex
+      real(c_double), value :: const_k2
+      real(c_double), intent(in) :: coef_nabk(*)
+      integer(c_int), value :: nzlma, maxmjj
+      integer(c_int), intent(in) :: mjj(*), jjp(*), iuv(*)
+      complex(c_double_complex), intent(in) :: uvk(*)
+    end function
+    integer(c_int) function rsdft_gpu_zhamiltonian(k, s, tpsi, htpsi, n1, n2, ib1, ib2) &
+         bind(C, name="rsdft_gpu_zhamiltonian")
+      import :: c_int, c_double_complex
+      integer(c_int), value :: k, s, n1, n2, ib1, ib2
+      complex(c_double_complex), intent(in) :: tpsi(*)
+      complex(c_double_complex), intent(out) :: htpsi(*)
+    end function
+    integer(c_int) function rsdft_gpu_zconjugate_gradient(unk, esp, res, ml0, mb) &
+         bind(C, name="rsdft_gpu_zconjugate_gradient")
+      import :: c_int, c_double, c_double_complex
+      complex(c_double_complex), intent(inout) :: unk(*)
+      real(c_double), intent(out) :: esp(*), res(*)
+      integer(c_int), value :: ml0, mb
+    end function
+    integer(c_int) function rsdft_gpu_zgram_schmidt(unk, ml0, mb) &
+         bind(C, name="rsdft_gpu_zgram_schmidt")
+      import :: c_int, c_double_complex
+      complex(c_double_complex), intent(inout) :: unk(*)
+      integer(c_int), value :: ml0, mb
+    end function
+    integer(c_int) function rsdft_gpu_zsubspace_diag(unk, esp, ml0, mb) &
+         bind(C, name="rsdft_gpu_zsubspace_diag")
+      import :: c_int, c_double, c_double_complex
+      complex(c_double_complex), intent(inout) :: unk(*)
+      real(c_double), intent(out) :: esp(*)
+      integer(c_int), value :: ml0, mb
+    end function
+    integer(c_int) function rsdft_gpu_zsweep(unk, esp, res, ml0, mb, nsweep) &
+         bind(C, name="rsdft_gpu_zsweep")
+      import :: c_int, c_double, c_double_complex
+      complex(c_double_complex), intent(inout) :: unk(*)
+      real(c_double), intent(out) :: esp(*), res(*)
+      integer(c_int), value :: ml0, mb, nsweep
+    end function
     integer(c_int) function rsdft_gpu_get_timers(t, n) bind(C, name="rsdft_gpu_get_timers")
       import :: c_int, c_double
       real(c_double), intent(inout) :: t(*)
