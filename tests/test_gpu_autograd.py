@@ -82,6 +82,10 @@ def test_gradcheck_along_random_directions(setup, seed):
         return BandEnergy.apply(v0 + t * dv, R0 + t * dR, dev, s, x)
 
     t = torch.zeros(1, dtype=torch.float64, requires_grad=True)
-    ok = torch.autograd.gradcheck(lambda tt: f(tt[0]), (t,), eps=1e-4, atol=1e-9, rtol=1e-6,
+    # central differences of E ~ 6.3e3 Ha with step 1e-4: the rounding of the
+    # two FP64 band-energy sums alone allows ~ 50 eps |E| / 1e-4 ~ 7e-7 Ha of
+    # difference from the exact derivative (the exact gradient is pinned
+    # against the oracle's density and force above)
+    ok = torch.autograd.gradcheck(lambda tt: f(tt[0]), (t,), eps=1e-4, atol=7e-7, rtol=1e-6,
                                   nondet_tol=0.0)
     assert ok
