@@ -8,9 +8,10 @@
 // for the update.  subspace_diag_la (subspace_diag_la_module.f90:85-206)
 // becomes H Psi (batched), one GEMM for Psi^T H Psi, the dsyr2k
 // symmetrisation, an FP64 eigensolve on the GPU and one rotation GEMM.
-// Plain GEMMs go to rocBLAS (atomics disabled, so results are
-// deterministic); the FP32-storage GEMMs are hand-written MFMA kernels
-// (FP32 or BF16 hi+lo products, FP64-flushed accumulation).
+// The tall A^T B products are hand-written split-K MFMA kernels (FP64, or
+// FP32 / BF16 hi+lo products with FP64-flushed accumulation); the FP64
+// rotations and the small n x n products go to rocBLAS (atomics disabled, so
+// results are deterministic).
 #include "common.h"
 
 #include <cmath>
@@ -24,6 +25,8 @@ int split_gemm_tn(const float *A, const float *B, int nb, long K, double alpha, 
                   int sym_result);
 template <class T>
 int split_gemm_nn(const float *A, const double *U, int nb, long M, float *C, int fp32);
+int mfma_gemm_tn_f64(const double *A, const double *B, int nb, long K, double alpha, double *C,
+                     int sym_result);
 
 template <class S, class D>
 __global__ void k_convert(const S *src, D *dst, long n)
@@ -576,23 +579,20 @@ static int refine_eigh(const double *H, int n, double *V, double *D, int *ok)
     return kOK;
 }
 
-// C(nb x nb) = alpha * A^T B, A and B (ML x nb) column-major, result FP64
-// (rocblas_dsyrk for A == B measured ~10x slower than dgemm at 480 x 373,248).
-// sym_result: A^T B is symmetric in exact arithmetic; the FP32 kernels then
-// compute the tile pairs i <= j only and mirror them
+// C(nb x nb) = alpha * A^T B, A and B (ML x nb) column-major, result FP64,
+// on the hand-written split-K MFMA kernels in both precisions (rocBLAS dgemm
+// computes the full product; dsyrk for A == B measured ~10x slower than dgemm
+// at 480 x 373,248).  sym_result (or A == B): A^T B is symmetric in exact
+// arithmetic; only the tile pairs i <= j are computed and mirrored
 template <class T>
 static int gemm_tn(const T *A, const T *B, int nb, double alpha, double *C,
                    bool sym_result = false)
 {
     Context &c = ctx();
     const long ML = c.g.ML;
-    if (sizeof(T) == 8) {
-        const double a = alpha, z = 0.0;
-        RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb,
-                              (int)ML, &a, (const double *)A, (int)ML, (const double *)B, (int)ML,
-                              &z, C, nb));
-        return kOK;
-    }
+    if (sizeof(T) == 8)
+        return mfma_gemm_tn_f64((const double *)A, (const double *)B, nb, ML, alpha, C,
+                                sym_result ? 1 : 0);
     return split_gemm_tn<T>((const float *)A, (const float *)B, nb, ML, alpha, C,
                             sym_result ? 1 : 0);
 }

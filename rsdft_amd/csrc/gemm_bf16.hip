@@ -26,12 +26,14 @@
 //
 // Lane layouts (checked on gfx950): a 16x16xK MFMA takes A[i][k] from lane
 // i + 16*(k/kpl) and B[k][j] from lane j + 16*(k/kpl); D[i][j] is register
-// i%4 of lane j + 16*(i/4).  Any permutation of k shared by A and B leaves
+// i%4 of lane j + 16*(i/4) for the FP32-output products and register i/4 of
+// lane j + 16*(i%4) for v_mfma_f64_16x16x4f64.  Any permutation of k shared by A and B leaves
 // the product unchanged, so every lane reads 8 consecutive k of its row from
 // LDS and the f32 mode walks them as 8 steps of k = 4.
 #include "common.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace rsdft {
 
@@ -64,13 +66,15 @@ __device__ __forceinline__ void lds_read8(const float *p, float (&x)[8]) {
     x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
 }
 
-template <int MODE, int BB>
-__device__ __forceinline__ void mfma_block(const float (&af)[BB][8], const float (&bf)[BB][8],
+// acc[bi][bj] += sum over the KPL k of this lane's A and B rows (KPL a
+// multiple of 8: the BF16 products take 8 k per lane and instruction)
+template <int MODE, int BB, int KPL>
+__device__ __forceinline__ void mfma_block(const float (&af)[BB][KPL], const float (&bf)[BB][KPL],
                                            v4f (&acc)[BB][BB])
 {
     if constexpr (MODE == 1) {
 #pragma unroll
-        for (int s = 0; s < 8; ++s)
+        for (int s = 0; s < KPL; ++s)
 #pragma unroll
             for (int bi = 0; bi < BB; ++bi)
 #pragma unroll
@@ -78,36 +82,67 @@ __device__ __forceinline__ void mfma_block(const float (&af)[BB][8], const float
                     acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[bi][s], bf[bj][s],
                                                                       acc[bi][bj], 0, 0, 0);
     } else {
-        v8bf ah[BB], al[BB], bh[BB], bl[BB];
+        static_assert(KPL % 8 == 0, "8 k per BF16 instruction");
 #pragma unroll
-        for (int b = 0; b < BB; ++b) {
-            split_bf16(af[b], ah[b], al[b]);
-            split_bf16(bf[b], bh[b], bl[b]);
-        }
+        for (int k0 = 0; k0 < KPL; k0 += 8) {
+            v8bf ah[BB], al[BB], bh[BB], bl[BB];
 #pragma unroll
-        for (int bi = 0; bi < BB; ++bi)
+            for (int b = 0; b < BB; ++b) {
+                float ta[8], tb[8];
 #pragma unroll
-            for (int bj = 0; bj < BB; ++bj) {
-                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], bl[bj], acc[bi][bj], 0, 0, 0);
-                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
-                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
-                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+                for (int e = 0; e < 8; ++e) {
+                    ta[e] = af[b][k0 + e];
+                    tb[e] = bf[b][k0 + e];
+                }
+                split_bf16(ta, ah[b], al[b]);
+                split_bf16(tb, bh[b], bl[b]);
             }
+#pragma unroll
+            for (int bi = 0; bi < BB; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < BB; ++bj) {
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], bl[bj], acc[bi][bj], 0, 0, 0);
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+                }
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // TN split-K: partial W[chunk][col*nb + row] over k in [chunk*kc, ...)
 // tile pairs (ti, tj) are enumerated from the work index; tile = 32*BB.
+// A stage is KB k of the TILE columns of A and B.  Its global loads are
+// issued two stages ahead into registers (two register sets), the LDS holds
+// two stages (written at the top of a stage, read after its one barrier), so
+// the loads of stages s+1 and s+2 are in flight while stage s runs on the
+// matrix cores.  The LDS buffer written at stage s was last read at stage
+// s-2, before every wave passed the barrier of stage s-1.
+// FP32 (MODE 1/2): KB = 32, FP32 accumulators flushed into FP64 every
+// GFLUSH stages.  FP64 (MODE 0): KB = 16, v_mfma_f64_16x16x4f64 with FP64
+// accumulators, no flush.
 // ---------------------------------------------------------------------------
-template <int MODE, int BB>
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <class T, int MODE, int BB>
 __global__ void __launch_bounds__(256)
-k_gemm_tn(const float *__restrict__ A, const float *__restrict__ B, long K, int nb, int nt,
-          int sym, int npairs, long kc, int nchunk, double *__restrict__ W)
+k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int nt, int sym,
+          int npairs, long kc, int nchunk, double *__restrict__ W)
 {
+    constexpr bool F64 = sizeof(T) == 8;
+    static_assert(F64 == (MODE == 0), "MODE 0 is the FP64 product");
     constexpr int TILE = 32 * BB;
-    __shared__ __attribute__((aligned(16))) float As[TILE * GLDK];
-    __shared__ __attribute__((aligned(16))) float Bs[TILE * GLDK];
+    constexpr int KB = F64 ? 16 : GKB;      // k per stage
+    constexpr int LDK = KB + 4;             // padded LDS row (elements)
+    constexpr int KPL = KB / 4;             // consecutive k per lane per stage
+    constexpr int QR = KB / 4;              // 4-element quads per column per stage
+    constexpr int QPT = TILE * QR / 256;    // quads per thread per matrix per stage
+    static_assert(QPT * 256 == TILE * QR, "whole quads per thread");
+    typedef T Q __attribute__((ext_vector_type(4)));
+    typedef typename std::conditional<F64, v4d, v4f>::type Acc;
+    __shared__ __attribute__((aligned(32))) T As[2][TILE * LDK];
+    __shared__ __attribute__((aligned(32))) T Bs[2][TILE * LDK];
     const long total = (long)gridDim.x;
     const long w = xcd_work(blockIdx.x, total);
     if (w >= (long)npairs * nchunk) return;
@@ -126,69 +161,113 @@ k_gemm_tn(const float *__restrict__ A, const float *__restrict__ B, long K, int 
     const int i0 = ti * TILE, j0 = tj * TILE;
     const long kbeg = (long)chunk * kc;
     const long kend = min(K, kbeg + kc);
+    const long nst = kend > kbeg ? (kend - kbeg + KB - 1) / KB : 0;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = wave & 1, wj = wave >> 1;
     const int ar = wi * 16 * BB, bc = wj * 16 * BB;
     const int lrow = lane & 15, lq = lane >> 4;
     const bool vec = (K & 3) == 0;
 
-    v4f acc[BB][BB];
-    double accd[BB][BB][4];
+    Acc acc[BB][BB];
+    double accd[F64 ? 1 : BB][F64 ? 1 : BB][4];
 #pragma unroll
     for (int bi = 0; bi < BB; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < BB; ++bj) {
-            acc[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int bj = 0; bj < BB; ++bj) acc[bi][bj] = Acc{0, 0, 0, 0};
+    if constexpr (!F64) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) accd[bi][bj][v] = 0.0;
-        }
-    int stage = 0;
-    for (long kb = kbeg; kb < kend; kb += GKB) {
-        // stage TILE columns x 32 k of A and B (BB float4 per thread each)
+        for (int bi = 0; bi < BB; ++bi)
 #pragma unroll
-        for (int it = 0; it < BB; ++it) {
+            for (int bj = 0; bj < BB; ++bj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) accd[bi][bj][v] = 0.0;
+    }
+
+    Q ra[2][QPT], rb[2][QPT];
+    // quad it of this thread: column f / QR, k offset 4 (f % QR) of the stage
+    auto gload = [&](Q (&qa)[QPT], Q (&qb)[QPT], long kb) {
+#pragma unroll
+        for (int it = 0; it < QPT; ++it) {
             const int f = tid + it * 256;
-            const int col = f >> 3, kq = (f & 7) * 4;
+            const int col = f / QR, kq = (f - col * QR) * 4;
             const long gk = kb + kq;
-            float4 va = make_float4(0.f, 0.f, 0.f, 0.f), vb = va;
             const int ga = i0 + col, gb = j0 + col;
+            Q va = Q{0, 0, 0, 0}, vb = Q{0, 0, 0, 0};
             if (vec && gk + 3 < kend) {
-                if (ga < nb) va = *reinterpret_cast<const float4 *>(A + (long)ga * K + gk);
-                if (gb < nb) vb = *reinterpret_cast<const float4 *>(B + (long)gb * K + gk);
+                if (ga < nb) va = *reinterpret_cast<const Q *>(A + (long)ga * K + gk);
+                if (gb < nb) vb = *reinterpret_cast<const Q *>(B + (long)gb * K + gk);
             } else {
-                float ta[4] = {0.f, 0.f, 0.f, 0.f}, tb[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
                     if (gk + e < kend) {
-                        if (ga < nb) ta[e] = A[(long)ga * K + gk + e];
-                        if (gb < nb) tb[e] = B[(long)gb * K + gk + e];
+                        if (ga < nb) va[e] = A[(long)ga * K + gk + e];
+                        if (gb < nb) vb[e] = B[(long)gb * K + gk + e];
                     }
-                va = make_float4(ta[0], ta[1], ta[2], ta[3]);
-                vb = make_float4(tb[0], tb[1], tb[2], tb[3]);
             }
-            *reinterpret_cast<float4 *>(&As[col * GLDK + kq]) = va;
-            *reinterpret_cast<float4 *>(&Bs[col * GLDK + kq]) = vb;
+            qa[it] = va;
+            qb[it] = vb;
         }
+    };
+    auto lstore = [&](const Q (&qa)[QPT], const Q (&qb)[QPT], int buf) {
+#pragma unroll
+        for (int it = 0; it < QPT; ++it) {
+            const int f = tid + it * 256;
+            const int col = f / QR, kq = (f - col * QR) * 4;
+            *reinterpret_cast<Q *>(&As[buf][col * LDK + kq]) = qa[it];
+            *reinterpret_cast<Q *>(&Bs[buf][col * LDK + kq]) = qb[it];
+        }
+    };
+    if (nst > 0) gload(ra[0], rb[0], kbeg);
+    if (nst > 1) gload(ra[1], rb[1], kbeg + KB);
+    int fl = 0;
+    auto stage = [&](auto set_tag, long s) {
+        constexpr int S = decltype(set_tag)::value;
+        lstore(ra[S], rb[S], S);
         __syncthreads();
-        float af[BB][8], bf[BB][8];
+        if (s + 2 < nst) gload(ra[S], rb[S], kbeg + (s + 2) * KB);
+        T af[BB][KPL], bf[BB][KPL];
 #pragma unroll
         for (int b = 0; b < BB; ++b) {
-            lds_read8(&As[(ar + b * 16 + lrow) * GLDK + 8 * lq], af[b]);
-            lds_read8(&Bs[(bc + b * 16 + lrow) * GLDK + 8 * lq], bf[b]);
-        }
-        __syncthreads();
-        mfma_block<MODE, BB>(af, bf, acc);
-        if (++stage == GFLUSH) {
-            stage = 0;
+            const T *pa = &As[S][(ar + b * 16 + lrow) * LDK + KPL * lq];
+            const T *pb = &Bs[S][(bc + b * 16 + lrow) * LDK + KPL * lq];
 #pragma unroll
-            for (int bi = 0; bi < BB; ++bi)
+            for (int e = 0; e < KPL; e += 4) {
+                const Q qa = *reinterpret_cast<const Q *>(pa + e);
+                const Q qb = *reinterpret_cast<const Q *>(pb + e);
 #pragma unroll
-                for (int bj = 0; bj < BB; ++bj) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) accd[bi][bj][v] += (double)acc[bi][bj][v];
-                    acc[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
+                for (int v = 0; v < 4; ++v) {
+                    af[b][e + v] = qa[v];
+                    bf[b][e + v] = qb[v];
                 }
+            }
         }
+        if constexpr (F64) {
+#pragma unroll
+            for (int st = 0; st < KPL; ++st)
+#pragma unroll
+                for (int bi = 0; bi < BB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < BB; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bi][st], bf[bj][st],
+                                                                          acc[bi][bj], 0, 0, 0);
+        } else {
+            mfma_block<MODE, BB, KPL>(af, bf, acc);
+            if (++fl == GFLUSH) {
+                fl = 0;
+#pragma unroll
+                for (int bi = 0; bi < BB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < BB; ++bj) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) accd[bi][bj][v] += (double)acc[bi][bj][v];
+                        acc[bi][bj] = Acc{0, 0, 0, 0};
+                    }
+            }
+        }
+    };
+    for (long s = 0; s < nst; s += 2) {
+        stage(std::integral_constant<int, 0>{}, s);
+        if (s + 1 < nst) stage(std::integral_constant<int, 1>{}, s + 1);
     }
     double *Wc = W + (long)chunk * nb * nb;
 #pragma unroll
@@ -198,9 +277,14 @@ k_gemm_tn(const float *__restrict__ A, const float *__restrict__ B, long K, int 
             const int col = j0 + bc + bj * 16 + lrow;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = i0 + ar + bi * 16 + 4 * lq + v;
-                if (row < nb && col < nb)
-                    Wc[(long)col * nb + row] = accd[bi][bj][v] + (double)acc[bi][bj][v];
+                // D[i][j] of a 16x16 block is item v of lane j + 16 q with
+                // i = 4 q + v (f32 16x16x4) or i = q + 4 v (f64 16x16x4)
+                const int row = i0 + ar + bi * 16 + (F64 ? lq + 4 * v : 4 * lq + v);
+                if (row < nb && col < nb) {
+                    double x = (double)acc[bi][bj][v];
+                    if constexpr (!F64) x += accd[bi][bj][v];
+                    Wc[(long)col * nb + row] = x;
+                }
             }
         }
 }
@@ -375,31 +459,40 @@ __global__ void k_to_f32(const double *src, float *dst, long n)
 // ---------------------------------------------------------------------------
 // sym_result: A^T B is known to be symmetric (A == B, or B = H A with H
 // symmetric): only the tile pairs i <= j are computed, the rest mirrored
-template <int MODE>
-static int gemm_tn_mode(const float *A, const float *B, int nb, long K, double alpha, double *C,
+template <class T, int MODE>
+static int gemm_tn_mode(const T *A, const T *B, int nb, long K, double alpha, double *C,
                         bool sym_result)
 {
     Context &c = ctx();
     constexpr int BB = 2, TILE = 32 * BB;
+    constexpr int KB = sizeof(T) == 8 ? 16 : GKB;
     const int nt = (nb + TILE - 1) / TILE;
     const bool sym = A == B || sym_result;
     const int npairs = sym ? nt * (nt + 1) / 2 : nt * nt;
     // enough workgroups to fill 256 CUs several times over
     int nchunk = (int)std::max<long>(1, std::min<long>((4096 + npairs - 1) / npairs, (K + 4095) / 4096));
     long kc = (K + nchunk - 1) / nchunk;
-    kc = (kc + GKB - 1) / GKB * GKB;
+    kc = (kc + KB - 1) / KB * KB;
     nchunk = (int)((K + kc - 1) / kc);
     RS_TRY(c.gemm_part.ensure(sizeof(double) * (size_t)nchunk * nb * nb));
     double *W = c.gemm_part.as<double>();
     long nwork = (long)npairs * nchunk;
     long grid = (nwork + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_gemm_tn<MODE, BB>), dim3(grid), dim3(256), 0, c.stream, A, B, K, nb, nt,
-                       sym ? 1 : 0, npairs, kc, nchunk, W);
+    hipLaunchKernelGGL((k_gemm_tn<T, MODE, BB>), dim3(grid), dim3(256), 0, c.stream, A, B, K, nb,
+                       nt, sym ? 1 : 0, npairs, kc, nchunk, W);
     RS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_gemm_tn_reduce, dim3((nb + 127) / 128, nb), dim3(128), 0, c.stream, W, nb,
                        nchunk, alpha, sym ? 1 : 0, TILE, C);
     RS_HIP(hipGetLastError());
     return kOK;
+}
+
+// FP64 A^T B on the FP64 matrix cores (same split-K tiling and fixed-order
+// reduction as the FP32 products)
+int mfma_gemm_tn_f64(const double *A, const double *B, int nb, long K, double alpha, double *C,
+                     int sym_result)
+{
+    return gemm_tn_mode<double, 0>(A, B, nb, K, alpha, C, sym_result != 0);
 }
 
 // mode 1: the rotation is a plain FP32 GEMM (K = nb is short), rocBLAS sgemm
@@ -443,8 +536,8 @@ template <class T>
 int split_gemm_tn(const float *A, const float *B, int nb, long K, double alpha, double *C,
                   int sym_result)
 {
-    return ctx().prec == 2 ? gemm_tn_mode<2>(A, B, nb, K, alpha, C, sym_result != 0)
-                           : gemm_tn_mode<1>(A, B, nb, K, alpha, C, sym_result != 0);
+    return ctx().prec == 2 ? gemm_tn_mode<float, 2>(A, B, nb, K, alpha, C, sym_result != 0)
+                           : gemm_tn_mode<float, 1>(A, B, nb, K, alpha, C, sym_result != 0);
 }
 
 // fp32 != 0: FP32 products whatever the precision mode
