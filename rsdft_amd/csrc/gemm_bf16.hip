@@ -169,11 +169,11 @@ k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int 
     const bool vec = (K & 3) == 0;
 
     Acc acc[BB][BB];
-    double accd[F64 ? 1 : BB][F64 ? 1 : BB][4];
 #pragma unroll
     for (int bi = 0; bi < BB; ++bi)
 #pragma unroll
         for (int bj = 0; bj < BB; ++bj) acc[bi][bj] = Acc{0, 0, 0, 0};
+    double accd[F64 ? 1 : BB][F64 ? 1 : BB][4];
     if constexpr (!F64) {
 #pragma unroll
         for (int bi = 0; bi < BB; ++bi)
@@ -219,8 +219,7 @@ k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int 
     };
     if (nst > 0) gload(ra[0], rb[0], kbeg);
     if (nst > 1) gload(ra[1], rb[1], kbeg + KB);
-    int fl = 0;
-    auto stage = [&](auto set_tag, long s) {
+    auto stage = [&](auto set_tag, long s, Acc (&ac)[BB][BB]) {
         constexpr int S = decltype(set_tag)::value;
         lstore(ra[S], rb[S], S);
         __syncthreads();
@@ -248,26 +247,41 @@ k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int 
                 for (int bi = 0; bi < BB; ++bi)
 #pragma unroll
                     for (int bj = 0; bj < BB; ++bj)
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bi][st], bf[bj][st],
-                                                                          acc[bi][bj], 0, 0, 0);
+                        ac[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bi][st], bf[bj][st],
+                                                                         ac[bi][bj], 0, 0, 0);
         } else {
-            mfma_block<MODE, BB, KPL>(af, bf, acc);
-            if (++fl == GFLUSH) {
-                fl = 0;
-#pragma unroll
-                for (int bi = 0; bi < BB; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < BB; ++bj) {
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) accd[bi][bj][v] += (double)acc[bi][bj][v];
-                        acc[bi][bj] = Acc{0, 0, 0, 0};
-                    }
-            }
+            mfma_block<MODE, BB, KPL>(af, bf, ac);
         }
     };
-    for (long s = 0; s < nst; s += 2) {
-        stage(std::integral_constant<int, 0>{}, s);
-        if (s + 1 < nst) stage(std::integral_constant<int, 1>{}, s + 1);
+    // GFLUSH (even) stages per group: the FP32 accumulators start at zero in
+    // every group and are added into the FP64 ones at its end, so they stay
+    // in the matrix-core registers for the whole group
+    static_assert(GFLUSH % 2 == 0, "stage parity repeats per group");
+    for (long s = 0; s < nst; s += GFLUSH) {
+        if constexpr (F64) {
+#pragma unroll
+            for (int u = 0; u < GFLUSH; u += 2) {
+                if (s + u < nst) stage(std::integral_constant<int, 0>{}, s + u, acc);
+                if (s + u + 1 < nst) stage(std::integral_constant<int, 1>{}, s + u + 1, acc);
+            }
+        } else {
+            Acc gacc[BB][BB];
+#pragma unroll
+            for (int bi = 0; bi < BB; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < BB; ++bj) gacc[bi][bj] = Acc{0, 0, 0, 0};
+#pragma unroll
+            for (int u = 0; u < GFLUSH; u += 2) {
+                if (s + u < nst) stage(std::integral_constant<int, 0>{}, s + u, gacc);
+                if (s + u + 1 < nst) stage(std::integral_constant<int, 1>{}, s + u + 1, gacc);
+            }
+#pragma unroll
+            for (int bi = 0; bi < BB; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < BB; ++bj)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) accd[bi][bj][v] += (double)gacc[bi][bj][v];
+        }
     }
     double *Wc = W + (long)chunk * nb * nb;
 #pragma unroll
@@ -281,8 +295,9 @@ k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int 
                 // i = 4 q + v (f32 16x16x4) or i = q + 4 v (f64 16x16x4)
                 const int row = i0 + ar + bi * 16 + (F64 ? lq + 4 * v : 4 * lq + v);
                 if (row < nb && col < nb) {
-                    double x = (double)acc[bi][bj][v];
-                    if constexpr (!F64) x += accd[bi][bj][v];
+                    double x;
+                    if constexpr (F64) x = acc[bi][bj][v];
+                    else x = accd[bi][bj][v];
                     Wc[(long)col * nb + row] = x;
                 }
             }
