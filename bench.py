@@ -38,7 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-MFMA_PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}    # dense MFMA, MI355X
+MFMA_PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6, "bf16": 2500.0}    # dense MFMA, MI355X
 # sweeps of the accuracy comparison (the FP64 Si216 run is 10 sweeps; the
 # mixed run replaces the first 8 by mixed-precision sweeps)
 NSWEEP = 10
@@ -311,8 +311,9 @@ def main():
     ap.add_argument("--bands", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--hpsi-reps", type=int, default=20)
-    ap.add_argument("--gemm", choices=["fp32", "bf16"], default="fp32",
-                    help="GEMM precision of the mixed path")
+    ap.add_argument("--gemm", choices=["fp32", "bf16"], default="bf16",
+                    help="GEMMs of the mixed path: bf16 = three-piece BF16 split (six "
+                         "v_mfma_f32_16x16x32_bf16 products per block), fp32 = FP32 MFMA")
     ap.add_argument("--pc-mode", type=int, choices=[0, 1], default=1,
                     help="preconditioner: 1 s-step (default), 0 step-by-step recurrence")
     ap.add_argument("--traffic-probe", action="store_true")
@@ -450,17 +451,23 @@ def main():
     sym_flops = ML * nb * (nb + 1)
     rot_flops = 2 * ML * nb * nb
 
-    def gemm_util(ms, peak):
+    def gemm_util(ms, peak, products=1):
+        """tflops: useful flops of the product; mfma_util: the matrix-core
+        work issued (products x flops: six BF16 piece products per FP32
+        product in the split mode) over the peak of those instructions."""
         fl = (sym_flops, sym_flops, rot_flops)
         names = ("overlap_xtx", "xt_hx", "rotation")
         return {n: {"ms": float(t), "tflops": f / (t * 1e-3) / 1e12,
-                    "mfma_util": f / (t * 1e-3) / 1e12 / peak} for n, t, f in zip(names, ms, fl)}
+                    "mfma_util": products * f / (t * 1e-3) / 1e12 / peak}
+                for n, t, f in zip(names, ms, fl)}
 
     # ---- sweep roofline denominators (sum of bytes/BW + flops/peak) ----------
     rej = float(timm["cg_rejected_bands"]) / nb
     led32 = sweep_ledger(sysm, nb, 4, rej)
     led64 = sweep_ledger(sysm, nb, 8, float(tim64["cg_rejected_bands"]) / nb)
-    roof32 = ledger_time_ms(led32, MFMA_PEAK_TFLOPS["f32"])
+    # the split mode's matrix-core ceiling per useful flop: six BF16 products
+    roof32 = ledger_time_ms(led32, MFMA_PEAK_TFLOPS["bf16"] / 6 if args.gemm == "bf16"
+                            else MFMA_PEAK_TFLOPS["f32"])
     roof64 = ledger_time_ms(led64, MFMA_PEAK_TFLOPS["f64"])
 
     cpu, esp_oracle = (None, None)
@@ -490,7 +497,8 @@ def main():
         "config": {
             "workload": f"{sysm.name}: diamond Si, {sysm.grid.N[0]}^3 grid, {nb} bands, Md={sysm.Md}, "
                         f"{sysm.proj.nzlma} s+p KB projectors; CG(Ncg=3,mloop=3)+GS+subspace sweep",
-            "mixed_gemm": args.gemm,
+            "mixed_gemm": "bf16x3 (three-piece BF16 split, six products)" if args.gemm == "bf16"
+                          else "fp32",
             "fp64_s_per_sweep": t_fp64,
             "fp64_over_mixed": t_fp64 / t_mix,
             "accuracy_protocol": f"{NSWEEP} FP64 sweeps vs {NMIXED} mixed + {NSWEEP - NMIXED} FP64 "
@@ -520,7 +528,8 @@ def main():
                 ("hamiltonian", "kinetic", "localpot", "nonlocal", "stencil", "projection",
                  "nl_stage", "copy"), hpm)},
             "copy_gbs_fp32": copy_gbs,
-            "gemm_fp32": gemm_util(gmm, MFMA_PEAK_TFLOPS["f32"]),
+            "gemm_mixed": (gemm_util(gmm, MFMA_PEAK_TFLOPS["bf16"], 6) if args.gemm == "bf16"
+                           else gemm_util(gmm, MFMA_PEAK_TFLOPS["f32"])),
             "gemm_fp64": gemm_util(gm64, MFMA_PEAK_TFLOPS["f64"]),
             "sweep_roofline": {
                 "mixed_ms": roof32, "mixed_frac": roof32 / (t_mix * 1e3),

@@ -24,7 +24,7 @@ template <class T>
 int split_gemm_tn(const float *A, const float *B, int nb, long K, double alpha, double *C,
                   int sym_result);
 template <class T>
-int split_gemm_nn(const float *A, const double *U, int nb, long M, float *C, int fp32);
+int split_gemm_nn(const float *A, const double *U, int nb, long M, float *C);
 int mfma_gemm_tn_f64(const double *A, const double *B, int nb, long K, double alpha, double *C,
                      int sym_result);
 
@@ -597,10 +597,9 @@ static int gemm_tn(const T *A, const T *B, int nb, double alpha, double *C,
                             sym_result ? 1 : 0);
 }
 
-// Out(ML x nb) = A(ML x nb) * U(nb x nb), U FP64; fp32: FP32 products in the
-// BF16 split mode too
+// Out(ML x nb) = A(ML x nb) * U(nb x nb), U FP64
 template <class T>
-static int gemm_nn(const T *A, const double *U, int nb, T *Out, int fp32 = 0)
+static int gemm_nn(const T *A, const double *U, int nb, T *Out)
 {
     Context &c = ctx();
     const long ML = c.g.ML;
@@ -611,7 +610,7 @@ static int gemm_nn(const T *A, const double *U, int nb, T *Out, int fp32 = 0)
                               (int)ML));
         return kOK;
     }
-    return split_gemm_nn<T>((const float *)A, U, nb, ML, (float *)Out, fp32);
+    return split_gemm_nn<T>((const float *)A, U, nb, ML, (float *)Out);
 }
 
 static int ensure_dense(int nb)
@@ -748,10 +747,8 @@ int rayleigh_ritz_factored_dev(T *X, const T *HX, int nb, double *esp_dev)
     // C <- R^-1 V
     RS_BLAS(rocblas_dgemm(c.blas, rocblas_operation_none, rocblas_operation_none, nb, nb, nb, &one,
                           Ri, nb, Vec, nb, &zero, C, nb));
-    // FP32 products: an error of the rotation here is not absorbed by a later
-    // Rayleigh-Ritz step (the BF16 split product of the two-step path is)
     T *Out = c.tmp.as<T>();
-    RS_TRY(gemm_nn<T>(X, C, nb, Out, 1));
+    RS_TRY(gemm_nn<T>(X, C, nb, Out));
     RS_HIP(hipMemcpyAsync(esp_dev, D, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
     std::swap(c.psi.p, c.tmp.p);
     std::swap(c.psi.bytes, c.tmp.bytes);
@@ -780,7 +777,7 @@ static int time_gemms_t(int nrep, double *ms)
             if (rep == 0) RS_HIP(hipEventRecord(a, c.stream));
             if (t == 0) RS_TRY(gemm_tn<T>(X, X, nb, c.g.dV, S));
             else if (t == 1) RS_TRY(gemm_tn<T>(X, Y, nb, 1.0, C, true));
-            else RS_TRY(gemm_nn<T>(X, U, nb, Out, 1));
+            else RS_TRY(gemm_nn<T>(X, U, nb, Out));
         }
         RS_HIP(hipEventRecord(b, c.stream));
         RS_HIP(hipEventSynchronize(b));

@@ -11,10 +11,13 @@
 //
 // Two arithmetic modes on FP32 inputs:
 //   mode 1: v_mfma_f32_16x16x4_f32 products, FP32 accumulation;
-//   mode 2: BF16 hi+lo split, x = hi + lo with hi = bf16(x), lo = bf16(x - hi),
-//           v_mfma_f32_16x16x32_bf16 products with FP32 accumulation: all four
-//           hi/lo products in TN (lo*lo is a positive bias on diagonals such
-//           as |psi|^2 if dropped), hi*hi + hi*lo + lo*hi in NN.
+//   mode 2: BF16 three-piece split, x = hi + mid + lo (each a BF16 holding the
+//           next 8 bits of x), v_mfma_f32_16x16x32_bf16 products with FP32
+//           accumulation: the six piece products of order >= 2^-16 (hi*hi,
+//           hi*mid, mid*hi, hi*lo, mid*mid, lo*hi), so each product is good
+//           to ~2^-25 relative, the accuracy of the FP32 products of mode 1,
+//           at six BF16 instructions per 32 k where mode 1 takes eight FP32
+//           instructions of 16x the cycles per flop.
 // NN is applied in correction form, Out = A diag(U) + A (U - diag(U)): the
 // diagonal is an exact FP32 column scale and only the off-diagonal part goes
 // through the matrix cores, so the product error scales with the
@@ -41,7 +44,6 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr int GKB = 32;        // k per LDS stage
-constexpr int GLDK = GKB + 4;  // padded LDS row (floats)
 constexpr int GFLUSH = 4;      // stages between FP64 flushes in TN
 
 __device__ __forceinline__ long xcd_work(long L, long total) {
@@ -50,64 +52,39 @@ __device__ __forceinline__ long xcd_work(long L, long total) {
     return (L & 7) * per + (L >> 3);
 }
 
-__device__ __forceinline__ void split_bf16(const float (&x)[8], v8bf &hi, v8bf &lo) {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const __bf16 h = (__bf16)x[e];
-        hi[e] = h;
-        lo[e] = (__bf16)(x[e] - (float)h);
-    }
-}
+// x = hi + mid + lo with hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid):
+// both differences are exact in FP32, so the three pieces carry all 24 bits
+// of x (the residual left is below 2^-26 |x|).  The kernels below split each
+// element once, where it is staged into LDS.
 
-__device__ __forceinline__ void lds_read8(const float *p, float (&x)[8]) {
-    const float4 a = *reinterpret_cast<const float4 *>(p);
-    const float4 b = *reinterpret_cast<const float4 *>(p + 4);
-    x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
-    x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
-}
-
-// acc[bi][bj] += sum over the KPL k of this lane's A and B rows (KPL a
-// multiple of 8: the BF16 products take 8 k per lane and instruction)
-template <int MODE, int BB, int KPL>
-__device__ __forceinline__ void mfma_block(const float (&af)[BB][KPL], const float (&bf)[BB][KPL],
-                                           v4f (&acc)[BB][BB])
+// acc += a * b over one 16x16x32 block from the three-piece splits: the six
+// products down to order 2^-16 relative, smallest first; the dropped
+// mid*lo, lo*mid and lo*lo terms are below 2^-25 of |a||b|
+__device__ __forceinline__ v4f mfma_split3(const v8bf &ah, const v8bf &am, const v8bf &al,
+                                           const v8bf &bh, const v8bf &bm, const v8bf &bl, v4f acc)
 {
-    if constexpr (MODE == 1) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, acc, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+}
+
+// acc[bi][bj] += sum over the KPL k of this lane's A and B rows, FP32
+// products (the lane's KPL consecutive k walked as KPL steps of k = 4)
+template <int BB, int KPL>
+__device__ __forceinline__ void mfma_block_f32(const float (&af)[BB][KPL],
+                                               const float (&bf)[BB][KPL], v4f (&acc)[BB][BB])
+{
 #pragma unroll
-        for (int s = 0; s < KPL; ++s)
+    for (int s = 0; s < KPL; ++s)
 #pragma unroll
-            for (int bi = 0; bi < BB; ++bi)
+        for (int bi = 0; bi < BB; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < BB; ++bj)
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[bi][s], bf[bj][s],
-                                                                      acc[bi][bj], 0, 0, 0);
-    } else {
-        static_assert(KPL % 8 == 0, "8 k per BF16 instruction");
-#pragma unroll
-        for (int k0 = 0; k0 < KPL; k0 += 8) {
-            v8bf ah[BB], al[BB], bh[BB], bl[BB];
-#pragma unroll
-            for (int b = 0; b < BB; ++b) {
-                float ta[8], tb[8];
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    ta[e] = af[b][k0 + e];
-                    tb[e] = bf[b][k0 + e];
-                }
-                split_bf16(ta, ah[b], al[b]);
-                split_bf16(tb, bh[b], bl[b]);
-            }
-#pragma unroll
-            for (int bi = 0; bi < BB; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < BB; ++bj) {
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], bl[bj], acc[bi][bj], 0, 0, 0);
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
-                }
-        }
-    }
+            for (int bj = 0; bj < BB; ++bj)
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[bi][s], bf[bj][s],
+                                                                  acc[bi][bj], 0, 0, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -141,8 +118,20 @@ k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int 
     static_assert(QPT * 256 == TILE * QR, "whole quads per thread");
     typedef T Q __attribute__((ext_vector_type(4)));
     typedef typename std::conditional<F64, v4d, v4f>::type Acc;
-    __shared__ __attribute__((aligned(32))) T As[2][TILE * LDK];
-    __shared__ __attribute__((aligned(32))) T Bs[2][TILE * LDK];
+    // MODE 2 stages the three BF16 pieces of every element (split once, when
+    // the loading thread stores it): piece arrays [col][k] with a 40-element
+    // (80-byte) pitch, in place of the FP32 copies of the other modes
+    constexpr bool SPL = MODE == 2;
+    constexpr int LDKB = KB + 8;
+    constexpr int PIECE = TILE * LDKB;   // bf16 elements per piece array
+    constexpr size_t STAGE_BYTES = SPL ? 2 * 3 * PIECE * sizeof(__bf16) : 2 * TILE * LDK * sizeof(T);
+    __shared__ __attribute__((aligned(32))) unsigned char lds_raw[2 * STAGE_BYTES];
+    T (*As)[TILE * LDK] = reinterpret_cast<T (*)[TILE * LDK]>(lds_raw);
+    T (*Bs)[TILE * LDK] = reinterpret_cast<T (*)[TILE * LDK]>(lds_raw + 2 * TILE * LDK * sizeof(T));
+    // piece q (0 hi, 1 mid, 2 lo) of matrix m (0 A, 1 B) in stage buffer b
+    auto piece = [&](int b, int m, int q) {
+        return reinterpret_cast<__bf16 *>(lds_raw + b * STAGE_BYTES) + (m * 3 + q) * PIECE;
+    };
     const long total = (long)gridDim.x;
     const long w = xcd_work(blockIdx.x, total);
     if (w >= (long)npairs * nchunk) return;
@@ -208,13 +197,34 @@ k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int 
             qb[it] = vb;
         }
     };
+    typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
     auto lstore = [&](const Q (&qa)[QPT], const Q (&qb)[QPT], int buf) {
 #pragma unroll
         for (int it = 0; it < QPT; ++it) {
             const int f = tid + it * 256;
             const int col = f / QR, kq = (f - col * QR) * 4;
-            *reinterpret_cast<Q *>(&As[buf][col * LDK + kq]) = qa[it];
-            *reinterpret_cast<Q *>(&Bs[buf][col * LDK + kq]) = qb[it];
+            if constexpr (SPL) {
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    const Q &x = m == 0 ? qa[it] : qb[it];
+                    v4bf h, md, lo;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const __bf16 hh = (__bf16)x[e];
+                        const float r = x[e] - (float)hh;
+                        const __bf16 mm = (__bf16)r;
+                        h[e] = hh;
+                        md[e] = mm;
+                        lo[e] = (__bf16)(r - (float)mm);
+                    }
+                    *reinterpret_cast<v4bf *>(piece(buf, m, 0) + col * LDKB + kq) = h;
+                    *reinterpret_cast<v4bf *>(piece(buf, m, 1) + col * LDKB + kq) = md;
+                    *reinterpret_cast<v4bf *>(piece(buf, m, 2) + col * LDKB + kq) = lo;
+                }
+            } else {
+                *reinterpret_cast<Q *>(&As[buf][col * LDK + kq]) = qa[it];
+                *reinterpret_cast<Q *>(&Bs[buf][col * LDK + kq]) = qb[it];
+            }
         }
     };
     if (nst > 0) gload(ra[0], rb[0], kbeg);
@@ -224,33 +234,54 @@ k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int 
         lstore(ra[S], rb[S], S);
         __syncthreads();
         if (s + 2 < nst) gload(ra[S], rb[S], kbeg + (s + 2) * KB);
-        T af[BB][KPL], bf[BB][KPL];
+        if constexpr (SPL) {
+            // lane (lrow, lq) holds k = 8 lq .. 8 lq + 7 of its row: the
+            // operand layout of v_mfma_f32_16x16x32_bf16 for the stage's 32 k
+            v8bf pa[3][BB], pb[3][BB];
 #pragma unroll
-        for (int b = 0; b < BB; ++b) {
-            const T *pa = &As[S][(ar + b * 16 + lrow) * LDK + KPL * lq];
-            const T *pb = &Bs[S][(bc + b * 16 + lrow) * LDK + KPL * lq];
+            for (int q = 0; q < 3; ++q)
 #pragma unroll
-            for (int e = 0; e < KPL; e += 4) {
-                const Q qa = *reinterpret_cast<const Q *>(pa + e);
-                const Q qb = *reinterpret_cast<const Q *>(pb + e);
+                for (int b = 0; b < BB; ++b) {
+                    pa[q][b] = *reinterpret_cast<const v8bf *>(
+                        piece(S, 0, q) + (ar + b * 16 + lrow) * LDKB + 8 * lq);
+                    pb[q][b] = *reinterpret_cast<const v8bf *>(
+                        piece(S, 1, q) + (bc + b * 16 + lrow) * LDKB + 8 * lq);
+                }
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    af[b][e + v] = qa[v];
-                    bf[b][e + v] = qb[v];
+            for (int bi = 0; bi < BB; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < BB; ++bj)
+                    ac[bi][bj] = mfma_split3(pa[0][bi], pa[1][bi], pa[2][bi], pb[0][bj],
+                                             pb[1][bj], pb[2][bj], ac[bi][bj]);
+        } else {
+            T af[BB][KPL], bf[BB][KPL];
+#pragma unroll
+            for (int b = 0; b < BB; ++b) {
+                const T *pa = &As[S][(ar + b * 16 + lrow) * LDK + KPL * lq];
+                const T *pb = &Bs[S][(bc + b * 16 + lrow) * LDK + KPL * lq];
+#pragma unroll
+                for (int e = 0; e < KPL; e += 4) {
+                    const Q qa = *reinterpret_cast<const Q *>(pa + e);
+                    const Q qb = *reinterpret_cast<const Q *>(pb + e);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        af[b][e + v] = qa[v];
+                        bf[b][e + v] = qb[v];
+                    }
                 }
             }
-        }
-        if constexpr (F64) {
+            if constexpr (F64) {
 #pragma unroll
-            for (int st = 0; st < KPL; ++st)
+                for (int st = 0; st < KPL; ++st)
 #pragma unroll
-                for (int bi = 0; bi < BB; ++bi)
+                    for (int bi = 0; bi < BB; ++bi)
 #pragma unroll
-                    for (int bj = 0; bj < BB; ++bj)
-                        ac[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bi][st], bf[bj][st],
-                                                                         ac[bi][bj], 0, 0, 0);
-        } else {
-            mfma_block<MODE, BB, KPL>(af, bf, ac);
+                        for (int bj = 0; bj < BB; ++bj)
+                            ac[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                af[bi][st], bf[bj][st], ac[bi][bj], 0, 0, 0);
+            } else {
+                mfma_block_f32<BB, KPL>(af, bf, ac);
+            }
         }
     };
     // GFLUSH (even) stages per group: the FP32 accumulators start at zero in
@@ -304,6 +335,187 @@ k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int 
         }
 }
 
+// ---------------------------------------------------------------------------
+// TN split-K in the three-piece BF16 mode on 128 x 128 tiles: 512 threads,
+// wave (wi, wj) = (w % 4, w / 4) owns rows 32 wi .. 32 wi + 31 and columns
+// 64 wj .. 64 wj + 63 of the tile (2 x 4 blocks of 16 x 16).  The six piece
+// products per block make the matrix cores fast enough that the operand
+// stream from L2 bounds the 64-wide tiles; doubling the tile halves the
+// operand bytes per product.  Global loads run two stages ahead in registers,
+// two LDS stages of split pieces, one barrier per stage (order below), FP64
+// flush every GFLUSH stages as in k_gemm_tn; the two LDS stages (120 KB) are
+// dynamic shared memory.
+// ---------------------------------------------------------------------------
+constexpr int kTn3Tile = 128;
+constexpr int kTn3Threads = 512;
+constexpr int kTn3Ldk = GKB + 8;   // bf16 pitch of a piece row
+constexpr size_t kTn3Stage = 2 * 3 * (size_t)kTn3Tile * kTn3Ldk * sizeof(__bf16);
+
+__global__ void __launch_bounds__(kTn3Threads)
+k_gemm_tn3(const float *__restrict__ A, const float *__restrict__ B, long K, int nb, int nt,
+           int sym, int npairs, long kc, int nchunk, double *__restrict__ W)
+{
+    constexpr int TILE = kTn3Tile, KB = GKB, LDKB = kTn3Ldk, PIECE = TILE * LDKB;
+    constexpr int QR = KB / 4;                      // 4-element quads per column per stage
+    constexpr int QPT = TILE * QR / kTn3Threads;    // quads per thread per matrix per stage
+    static_assert(QPT * kTn3Threads == TILE * QR, "whole quads per thread");
+    constexpr int BI = 2, BJ = 4;
+    typedef float Q __attribute__((ext_vector_type(4)));
+    typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_tn3[];
+    auto piece = [&](int b, int m, int q) {
+        return reinterpret_cast<__bf16 *>(lds_tn3 + b * kTn3Stage) + (m * 3 + q) * PIECE;
+    };
+    const long w = xcd_work(blockIdx.x, (long)gridDim.x);
+    if (w >= (long)npairs * nchunk) return;
+    const int chunk = (int)(w / npairs);
+    const int p = (int)(w - (long)chunk * npairs);
+    int ti, tj;
+    if (sym) {   // p enumerates i <= j column by column: p = j(j+1)/2 + i
+        tj = (int)((sqrtf(8.0f * p + 1.0f) - 1.0f) * 0.5f);
+        while ((tj + 1) * (tj + 2) / 2 <= p) ++tj;
+        while (tj * (tj + 1) / 2 > p) --tj;
+        ti = p - tj * (tj + 1) / 2;
+    } else {
+        tj = p / nt;
+        ti = p - tj * nt;
+    }
+    const int i0 = ti * TILE, j0 = tj * TILE;
+    const long kbeg = (long)chunk * kc;
+    const long kend = min(K, kbeg + kc);
+    const long nst = kend > kbeg ? (kend - kbeg + KB - 1) / KB : 0;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ar = (wave & 3) * 32, bc = (wave >> 2) * 64;
+    const int lrow = lane & 15, lq = lane >> 4;
+    const bool vec = (K & 3) == 0;
+
+    double accd[BI][BJ][4];
+#pragma unroll
+    for (int bi = 0; bi < BI; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < BJ; ++bj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) accd[bi][bj][v] = 0.0;
+
+    Q ra[2][QPT], rb[2][QPT];
+    auto gload = [&](Q (&qa)[QPT], Q (&qb)[QPT], long kb) {
+#pragma unroll
+        for (int it = 0; it < QPT; ++it) {
+            const int f = tid + it * kTn3Threads;
+            const int col = f / QR, kq = (f - col * QR) * 4;
+            const long gk = kb + kq;
+            const int ga = i0 + col, gb = j0 + col;
+            Q va = Q{0, 0, 0, 0}, vb = Q{0, 0, 0, 0};
+            if (vec && gk + 3 < kend) {
+                if (ga < nb) va = *reinterpret_cast<const Q *>(A + (long)ga * K + gk);
+                if (gb < nb) vb = *reinterpret_cast<const Q *>(B + (long)gb * K + gk);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (gk + e < kend) {
+                        if (ga < nb) va[e] = A[(long)ga * K + gk + e];
+                        if (gb < nb) vb[e] = B[(long)gb * K + gk + e];
+                    }
+            }
+            qa[it] = va;
+            qb[it] = vb;
+        }
+    };
+    auto lstore = [&](const Q (&qa)[QPT], const Q (&qb)[QPT], int buf) {
+#pragma unroll
+        for (int it = 0; it < QPT; ++it) {
+            const int f = tid + it * kTn3Threads;
+            const int col = f / QR, kq = (f - col * QR) * 4;
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const Q &x = m == 0 ? qa[it] : qb[it];
+                v4bf h, md, lo;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const __bf16 hh = (__bf16)x[e];
+                    const float r = x[e] - (float)hh;
+                    const __bf16 mm = (__bf16)r;
+                    h[e] = hh;
+                    md[e] = mm;
+                    lo[e] = (__bf16)(r - (float)mm);
+                }
+                *reinterpret_cast<v4bf *>(piece(buf, m, 0) + col * LDKB + kq) = h;
+                *reinterpret_cast<v4bf *>(piece(buf, m, 1) + col * LDKB + kq) = md;
+                *reinterpret_cast<v4bf *>(piece(buf, m, 2) + col * LDKB + kq) = lo;
+            }
+        }
+    };
+    // Stage s: after the barrier every wave reads its operands of stage s from
+    // LDS buffer s % 2, then splits and stores stage s + 1 (register set
+    // (s + 1) % 2) into the other buffer and refills that set with stage
+    // s + 3, while its matrix-core products of stage s run: the VALU split
+    // overlaps the MFMA work of the same wave.  The buffer written at stage s
+    // was last read at stage s - 1, before every wave passed this barrier.
+    if (nst > 0) gload(ra[0], rb[0], kbeg);
+    if (nst > 1) gload(ra[1], rb[1], kbeg + KB);
+    if (nst > 0) {
+        lstore(ra[0], rb[0], 0);
+        if (nst > 2) gload(ra[0], rb[0], kbeg + 2 * KB);
+    }
+    auto stage = [&](auto set_tag, long s, v4f (&ac)[BI][BJ]) {
+        constexpr int S = decltype(set_tag)::value;   // s % 2
+        __syncthreads();
+        v8bf pa[3][BI], pb[3][BJ];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+#pragma unroll
+            for (int b = 0; b < BI; ++b)
+                pa[q][b] = *reinterpret_cast<const v8bf *>(piece(S, 0, q) +
+                                                           (ar + b * 16 + lrow) * LDKB + 8 * lq);
+#pragma unroll
+            for (int b = 0; b < BJ; ++b)
+                pb[q][b] = *reinterpret_cast<const v8bf *>(piece(S, 1, q) +
+                                                           (bc + b * 16 + lrow) * LDKB + 8 * lq);
+        }
+        if (s + 1 < nst) {
+            lstore(ra[1 - S], rb[1 - S], 1 - S);
+            if (s + 3 < nst) gload(ra[1 - S], rb[1 - S], kbeg + (s + 3) * KB);
+        }
+#pragma unroll
+        for (int bi = 0; bi < BI; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < BJ; ++bj)
+                ac[bi][bj] = mfma_split3(pa[0][bi], pa[1][bi], pa[2][bi], pb[0][bj], pb[1][bj],
+                                         pb[2][bj], ac[bi][bj]);
+    };
+    static_assert(GFLUSH % 2 == 0, "stage parity repeats per group");
+    for (long s = 0; s < nst; s += GFLUSH) {
+        v4f gacc[BI][BJ];
+#pragma unroll
+        for (int bi = 0; bi < BI; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < BJ; ++bj) gacc[bi][bj] = v4f{0, 0, 0, 0};
+#pragma unroll
+        for (int u = 0; u < GFLUSH; u += 2) {
+            if (s + u < nst) stage(std::integral_constant<int, 0>{}, s + u, gacc);
+            if (s + u + 1 < nst) stage(std::integral_constant<int, 1>{}, s + u + 1, gacc);
+        }
+#pragma unroll
+        for (int bi = 0; bi < BI; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < BJ; ++bj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) accd[bi][bj][v] += (double)gacc[bi][bj][v];
+    }
+    double *Wc = W + (long)chunk * nb * nb;
+#pragma unroll
+    for (int bi = 0; bi < BI; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < BJ; ++bj) {
+            const int col = j0 + bc + bj * 16 + lrow;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = i0 + ar + bi * 16 + 4 * lq + v;
+                if (row < nb && col < nb) Wc[(long)col * nb + row] = accd[bi][bj][v];
+            }
+        }
+}
+
 // C(i,j) = alpha * sum_chunk W[chunk](i,j); symmetric: tiles below the
 // diagonal were not computed, take them from the mirrored element.
 __global__ void k_gemm_tn_reduce(const double *__restrict__ W, int nb, int nchunk, double alpha,
@@ -321,32 +533,62 @@ __global__ void k_gemm_tn_reduce(const double *__restrict__ W, int nb, int nchun
 }
 
 // ---------------------------------------------------------------------------
-// NN: Out = A * U, A (M x nb) ld M, U (nb x nb) FP32 ld nb.  Tile 64 rows x
-// 32*BN columns; wave (wr, wn) owns 32 x 16*BN.  Output staged through LDS
-// and stored as float4 runs along the rows.
+// NN (BF16 three-piece mode): Out = A * U, A (M x nb) ld M, U (nb x nb) FP64
+// ld nb, in correction form Out = A diag(U) + A E, E = U - diag(U).
+// k_split_u3 rounds E to FP32 and splits it once into its three BF16 pieces,
+// [col][k] with k padded to a multiple of 32 (zeros), so the GEMM copies them
+// into LDS without arithmetic.  Tile kNnRows rows x 32*BN columns, one wave
+// per 32 rows x 16*BN columns.  Every k step the loading thread splits its
+// 2 x 4 elements of A once and stores the pieces [row][k] (one 32-bit store
+// per row and piece: two consecutive k); the lanes then read 8 consecutive k
+// of their row or column per piece, the v_mfma_f32_16x16x32_bf16 operand
+// layout.  The next step's A and U are loaded into registers while the
+// current step is read and multiplied.  The output is staged through LDS and
+// stored as float4 runs along the rows, with the diagonal term A diag(U)
+// added in FP32.
 // ---------------------------------------------------------------------------
-// dst = src with a zero diagonal, diag = the diagonal, both FP32
-__global__ void k_split_diag_f32(const double *__restrict__ src, float *__restrict__ dst,
-                                 float *__restrict__ diag, int n)
+constexpr int kNnRows = 64;                    // rows of A per NN tile
+constexpr int kNnThreads = 2 * 64 * (kNnRows / 32);   // row groups of 32 x 2 column groups
+
+__global__ void k_split_u3(const double *__restrict__ U, int nb, int kpad,
+                           __bf16 *__restrict__ piece, float *__restrict__ diag)
 {
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (t >= (long)n * n) return;
-    const int j = (int)(t / n), i = (int)(t - (long)j * n);
-    dst[t] = i == j ? 0.f : (float)src[t];
-    if (i == j) diag[j] = (float)src[t];
+    const long n = (long)nb * kpad;
+    if (t >= n) return;
+    const int col = (int)(t / kpad), k = (int)(t - (long)col * kpad);
+    float x = 0.f;
+    if (k < nb) {
+        const float u = (float)U[(long)col * nb + k];
+        if (k == col) diag[col] = u;
+        else x = u;
+    }
+    const __bf16 h = (__bf16)x;
+    const float r = x - (float)h;
+    const __bf16 m = (__bf16)r;
+    piece[t] = h;
+    piece[n + t] = m;
+    piece[2 * n + t] = (__bf16)(r - (float)m);
 }
 
-template <int MODE, int BN>
-__global__ void __launch_bounds__(256)
-k_gemm_nn(const float *__restrict__ A, const float *__restrict__ U, long M, int nb, int ncolb,
-          const float *__restrict__ diag, float *__restrict__ Out)
+template <int BN>
+__global__ void __launch_bounds__(kNnThreads)
+k_gemm_nn(const float *__restrict__ A, const __bf16 *__restrict__ Up, int kpad, long M, int nb,
+          int ncolb, const float *__restrict__ diag, float *__restrict__ Out)
 {
-    constexpr int TM = 64, TN = 32 * BN;
-    constexpr int SA = TM * GLDK, SB = TN * GLDK;
-    constexpr int SC = TN * (TM + 4);
-    constexpr int SMEM = (SA + SB) > SC ? (SA + SB) : SC;
-    __shared__ __attribute__((aligned(16))) float sm[SMEM];
-    float *As = sm, *Bs = sm + SA, *Cs = sm;
+    constexpr int TM = kNnRows, TN = 32 * BN, NT = kNnThreads;
+    constexpr int LDKB = GKB + 8;                      // bf16 pitch of a piece row
+    constexpr int PA = TM * LDKB, PB = TN * LDKB;      // elements per piece array
+    constexpr int SIN = 3 * (PA + PB) * 2;             // bytes of the staged pieces
+    constexpr int SC = TN * (TM + 4) * 4;              // bytes of the output staging
+    constexpr int SMEM = SIN > SC ? SIN : SC;
+    constexpr int NU = 3 * TN * 4;                     // 16-byte U piece copies per k step
+    constexpr int UPT = (NU + NT - 1) / NT;            // per thread
+    static_assert(TM / 4 * (GKB / 2) == NT, "one row quad x k pair of A per thread");
+    __shared__ __attribute__((aligned(16))) unsigned char sm[SMEM];
+    __bf16 *Ap = reinterpret_cast<__bf16 *>(sm);       // piece q at Ap + q * PA
+    __bf16 *Bp = Ap + 3 * PA;                          // piece q at Bp + q * PB
+    float *Cs = reinterpret_cast<float *>(sm);
     const long total = (long)gridDim.x;
     const long nrowb = (M + TM - 1) / TM;
     const long w = xcd_work(blockIdx.x, total);
@@ -356,9 +598,12 @@ k_gemm_nn(const float *__restrict__ A, const float *__restrict__ U, long M, int 
     const long r0 = rb * TM;
     const int n0 = cb * TN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = wave & 1, wn = wave >> 1;
+    const int wr = wave % (TM / 32), wn = wave / (TM / 32);   // rows 32 wr, columns 16 BN wn
     const int lrow = lane & 15, lq = lane >> 4;
     const bool vec = (M & 3) == 0 && r0 + TM <= M;
+    const long pn = (long)nb * kpad;                   // elements per piece of U
+    // A loads: thread t owns rows rq .. rq+3 at k = kb + kp, kb + kp + 1
+    const int rq = (tid % (TM / 4)) * 4, kp = (tid / (TM / 4)) * 2;
 
     v4f acc[2][BN];
 #pragma unroll
@@ -366,67 +611,98 @@ k_gemm_nn(const float *__restrict__ A, const float *__restrict__ U, long M, int 
 #pragma unroll
         for (int bj = 0; bj < BN; ++bj) acc[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
 
-    for (int kb = 0; kb < nb; kb += GKB) {
-        // A block: 32 k x 64 rows, float4 along rows (2 per thread)
+    float ta[2][4];
+    uint4 tu[UPT];
+    // global operands of the k step starting at kb into registers
+    auto gload = [&](int kb) {
 #pragma unroll
-        for (int it = 0; it < 2; ++it) {
-            const int f = tid + it * 256;
-            const int k = f >> 4, rq = (f & 15) * 4;
-            const int gk = kb + k;
-            float t[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < 2; ++j) {
+            const int gk = kb + kp + j;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) ta[j][e] = 0.f;
             if (gk < nb) {
                 const float *src = A + (long)gk * M + r0 + rq;
                 if (vec) {
                     const float4 v = *reinterpret_cast<const float4 *>(src);
-                    t[0] = v.x; t[1] = v.y; t[2] = v.z; t[3] = v.w;
+                    ta[j][0] = v.x; ta[j][1] = v.y; ta[j][2] = v.z; ta[j][3] = v.w;
                 } else {
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
-                        if (r0 + rq + e < M) t[e] = src[e];
+                        if (r0 + rq + e < M) ta[j][e] = src[e];
                 }
             }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) As[(rq + e) * GLDK + k] = t[e];
         }
-        // U block: TN columns x 32 k, float (U is column-major, k contiguous)
-        for (int f = tid; f < TN * GKB; f += 256) {
-            const int col = f / GKB, k = f - col * GKB;
-            const int gn = n0 + col, gk = kb + k;
-            Bs[col * GLDK + k] = (gn < nb && gk < nb) ? U[(long)gn * nb + gk] : 0.f;
+#pragma unroll
+        for (int u = 0; u < UPT; ++u) {
+            const int f = tid + u * NT;
+            tu[u] = make_uint4(0u, 0u, 0u, 0u);
+            if (f < NU) {
+                const int q = f / (TN * 4), rem = f - q * (TN * 4);
+                const int col = rem >> 2, c8 = (rem & 3) * 8;
+                const int gn = n0 + col;
+                if (gn < nb)
+                    tu[u] = *reinterpret_cast<const uint4 *>(Up + q * pn + (long)gn * kpad + kb + c8);
+            }
+        }
+    };
+    gload(0);
+    for (int kb = 0; kb < nb; kb += GKB) {
+        // split the staged A elements (one 32-bit store per row and piece:
+        // two consecutive k) and store the U pieces
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            __bf16 pc[3][2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const float x = ta[j][e];
+                const __bf16 h = (__bf16)x;
+                const float r = x - (float)h;
+                const __bf16 m = (__bf16)r;
+                pc[0][j] = h;
+                pc[1][j] = m;
+                pc[2][j] = (__bf16)(r - (float)m);
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+                *reinterpret_cast<v2bf *>(Ap + q * PA + (rq + e) * LDKB + kp) =
+                    v2bf{pc[q][0], pc[q][1]};
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UPT; ++u) {
+            const int f = tid + u * NT;
+            if (f < NU) {
+                const int q = f / (TN * 4), rem = f - q * (TN * 4);
+                const int col = rem >> 2, c8 = (rem & 3) * 8;
+                *reinterpret_cast<uint4 *>(Bp + q * PB + col * LDKB + c8) = tu[u];
+            }
         }
         __syncthreads();
-        float af[2][8], bf[BN][8];
+        // the next step's operands are in flight while this step's are read
+        // and multiplied
+        if (kb + GKB < nb) gload(kb + GKB);
+        v8bf pa[3][2], pb[3][BN];
 #pragma unroll
-        for (int b = 0; b < 2; ++b) lds_read8(&As[(wr * 32 + b * 16 + lrow) * GLDK + 8 * lq], af[b]);
+        for (int q = 0; q < 3; ++q) {
 #pragma unroll
-        for (int b = 0; b < BN; ++b) lds_read8(&Bs[(wn * 16 * BN + b * 16 + lrow) * GLDK + 8 * lq], bf[b]);
-        __syncthreads();
-        if constexpr (MODE == 1) {
+            for (int b = 0; b < 2; ++b)
+                pa[q][b] = *reinterpret_cast<const v8bf *>(
+                    Ap + q * PA + (wr * 32 + b * 16 + lrow) * LDKB + 8 * lq);
 #pragma unroll
-            for (int s = 0; s < 8; ++s)
-#pragma unroll
-                for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < BN; ++bj)
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[bi][s], bf[bj][s],
-                                                                          acc[bi][bj], 0, 0, 0);
-        } else {
-            v8bf ah[2], al[2], bh[BN], bl[BN];
-#pragma unroll
-            for (int b = 0; b < 2; ++b) split_bf16(af[b], ah[b], al[b]);
-#pragma unroll
-            for (int b = 0; b < BN; ++b) split_bf16(bf[b], bh[b], bl[b]);
-#pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < BN; ++bj) {
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
-                }
+            for (int b = 0; b < BN; ++b)
+                pb[q][b] = *reinterpret_cast<const v8bf *>(
+                    Bp + q * PB + (wn * 16 * BN + b * 16 + lrow) * LDKB + 8 * lq);
         }
+        __syncthreads();
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < BN; ++bj)
+                acc[bi][bj] = mfma_split3(pa[0][bi], pa[1][bi], pa[2][bi], pb[0][bj], pb[1][bj],
+                                          pb[2][bj], acc[bi][bj]);
     }
-    // stage the 64 x TN tile column-major in LDS, then store float4 runs
+    // stage the TM x TN tile column-major in LDS, then store float4 runs
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -439,25 +715,23 @@ k_gemm_nn(const float *__restrict__ A, const float *__restrict__ U, long M, int 
             }
         }
     __syncthreads();
-    for (int f = tid; f < TN * (TM / 4); f += 256) {
-        const int c = f / (TM / 4), rq = (f - c * (TM / 4)) * 4;
+    for (int f = tid; f < TN * (TM / 4); f += NT) {
+        const int c = f / (TM / 4), rq4 = (f - c * (TM / 4)) * 4;
         const int gn = n0 + c;
         if (gn >= nb) continue;
-        const float *s = &Cs[c * (TM + 4) + rq];
-        float *d = Out + (long)gn * M + r0 + rq;
-        const float *a = A + (long)gn * M + r0 + rq;
+        const float *s = &Cs[c * (TM + 4) + rq4];
+        float *d = Out + (long)gn * M + r0 + rq4;
+        const float *a = A + (long)gn * M + r0 + rq4;
+        const float dg = diag[gn];
         if (vec) {
-            float4 o = make_float4(s[0], s[1], s[2], s[3]);
-            if (diag) {
-                const float dg = diag[gn];
-                const float4 av = *reinterpret_cast<const float4 *>(a);
-                o.x += dg * av.x; o.y += dg * av.y; o.z += dg * av.z; o.w += dg * av.w;
-            }
+            const float4 av = *reinterpret_cast<const float4 *>(a);
+            const float4 o = make_float4(s[0] + dg * av.x, s[1] + dg * av.y, s[2] + dg * av.z,
+                                         s[3] + dg * av.w);
             *reinterpret_cast<float4 *>(d) = o;
         } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                if (r0 + rq + e < M) d[e] = s[e] + (diag ? diag[gn] * a[e] : 0.f);
+                if (r0 + rq4 + e < M) d[e] = s[e] + dg * a[e];
         }
     }
 }
@@ -474,6 +748,40 @@ __global__ void k_to_f32(const double *src, float *dst, long n)
 // ---------------------------------------------------------------------------
 // sym_result: A^T B is known to be symmetric (A == B, or B = H A with H
 // symmetric): only the tile pairs i <= j are computed, the rest mirrored
+// three-piece BF16 mode: the 128-wide tile kernel
+static int gemm_tn_split3(const float *A, const float *B, int nb, long K, double alpha,
+                          double *C, bool sym_result)
+{
+    Context &c = ctx();
+    constexpr int TILE = kTn3Tile;
+    const int nt = (nb + TILE - 1) / TILE;
+    const bool sym = A == B || sym_result;
+    const int npairs = sym ? nt * (nt + 1) / 2 : nt * nt;
+    // about five rounds of one 512-thread workgroup per CU
+    int nchunk = (int)std::max<long>(1, std::min<long>((1280 + npairs - 1) / npairs, (K + 1023) / 1024));
+    long kc = (K + nchunk - 1) / nchunk;
+    kc = (kc + GKB - 1) / GKB * GKB;
+    nchunk = (int)((K + kc - 1) / kc);
+    RS_TRY(c.gemm_part.ensure(sizeof(double) * (size_t)nchunk * nb * nb));
+    double *W = c.gemm_part.as<double>();
+    static bool lds_set = false;
+    if (!lds_set) {
+        RS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_gemm_tn3),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(2 * kTn3Stage)));
+        lds_set = true;
+    }
+    const long nwork = (long)npairs * nchunk;
+    const long grid = (nwork + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_gemm_tn3, dim3(grid), dim3(kTn3Threads), 2 * kTn3Stage, c.stream, A, B,
+                       K, nb, nt, sym ? 1 : 0, npairs, kc, nchunk, W);
+    RS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_gemm_tn_reduce, dim3((nb + 127) / 128, nb), dim3(128), 0, c.stream, W, nb,
+                       nchunk, alpha, sym ? 1 : 0, TILE, C);
+    RS_HIP(hipGetLastError());
+    return kOK;
+}
+
 template <class T, int MODE>
 static int gemm_tn_mode(const T *A, const T *B, int nb, long K, double alpha, double *C,
                         bool sym_result)
@@ -526,23 +834,25 @@ static int gemm_nn_sgemm(const float *A, const double *U, int nb, long M, float 
     return kOK;
 }
 
-template <int MODE>
-static int gemm_nn_mode(const float *A, const double *U, int nb, long M, float *Out)
+// mode 2: the three-piece BF16 kernel above
+static int gemm_nn_split3(const float *A, const double *U, int nb, long M, float *Out)
 {
     Context &c = ctx();
-    if constexpr (MODE == 1) return gemm_nn_sgemm(A, U, nb, M, Out);
     constexpr int BN = 3, TN = 32 * BN;
-    RS_TRY(c.matT.ensure(sizeof(float) * ((size_t)nb * nb + nb)));
-    float *U32 = c.matT.as<float>();
-    float *D32 = U32 + (size_t)nb * nb;
-    const long nn = (long)nb * nb;
-    hipLaunchKernelGGL(k_split_diag_f32, dim3((nn + 255) / 256), dim3(256), 0, c.stream, U, U32, D32, nb);
+    const int kpad = (nb + GKB - 1) / GKB * GKB;
+    const long pn = (long)nb * kpad;
+    RS_TRY(c.matT.ensure(sizeof(__bf16) * 3 * (size_t)pn + sizeof(float) * (size_t)nb + 16));
+    __bf16 *Up = c.matT.as<__bf16>();
+    float *D32 = reinterpret_cast<float *>(Up + 3 * pn + 8 - (3 * pn) % 8);
+    hipLaunchKernelGGL(k_split_u3, dim3((unsigned)((pn + 255) / 256)), dim3(256), 0, c.stream, U,
+                       nb, kpad, Up, D32);
+    RS_HIP(hipGetLastError());
     const int ncolb = (nb + TN - 1) / TN;
-    const long nrowb = (M + 63) / 64;
+    const long nrowb = (M + kNnRows - 1) / kNnRows;
     const long nwork = nrowb * ncolb;
     const long grid = (nwork + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_gemm_nn<MODE, BN>), dim3(grid), dim3(256), 0, c.stream, A, U32, M, nb,
-                       ncolb, D32, Out);
+    hipLaunchKernelGGL((k_gemm_nn<BN>), dim3(grid), dim3(kNnThreads), 0, c.stream, A, Up, kpad, M,
+                       nb, ncolb, D32, Out);
     RS_HIP(hipGetLastError());
     return kOK;
 }
@@ -551,21 +861,19 @@ template <class T>
 int split_gemm_tn(const float *A, const float *B, int nb, long K, double alpha, double *C,
                   int sym_result)
 {
-    return ctx().prec == 2 ? gemm_tn_mode<float, 2>(A, B, nb, K, alpha, C, sym_result != 0)
+    return ctx().prec == 2 ? gemm_tn_split3(A, B, nb, K, alpha, C, sym_result != 0)
                            : gemm_tn_mode<float, 1>(A, B, nb, K, alpha, C, sym_result != 0);
 }
 
-// fp32 != 0: FP32 products whatever the precision mode
 template <class T>
-int split_gemm_nn(const float *A, const double *U, int nb, long M, float *Out, int fp32)
+int split_gemm_nn(const float *A, const double *U, int nb, long M, float *Out)
 {
-    return ctx().prec == 2 && !fp32 ? gemm_nn_mode<2>(A, U, nb, M, Out)
-                                    : gemm_nn_mode<1>(A, U, nb, M, Out);
+    return ctx().prec == 2 ? gemm_nn_split3(A, U, nb, M, Out) : gemm_nn_sgemm(A, U, nb, M, Out);
 }
 
 template int split_gemm_tn<float>(const float *, const float *, int, long, double, double *, int);
 template int split_gemm_tn<double>(const float *, const float *, int, long, double, double *, int);
-template int split_gemm_nn<float>(const float *, const double *, int, long, float *, int);
-template int split_gemm_nn<double>(const float *, const double *, int, long, float *, int);
+template int split_gemm_nn<float>(const float *, const double *, int, long, float *);
+template int split_gemm_nn<double>(const float *, const double *, int, long, float *);
 
 }  // namespace rsdft
