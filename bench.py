@@ -254,13 +254,14 @@ def traffic_probe():
 
 
 def sweep_probe():
-    """Two mixed-precision Si216 sweeps after one warm-up sweep (the counter
-    runs of the sweep's kernels profile this)."""
+    """Two mixed-precision Si216 sweeps (FP32 storage, three-piece BF16 GEMMs)
+    after one warm-up sweep (the counter runs of the sweep's kernels profile
+    this)."""
     from rsdft_amd.system import make_config
-    from rsdft_amd.gpu import RsdftGpu, PREC_MIXED_FP32
+    from rsdft_amd.gpu import RsdftGpu, PREC_MIXED_BF16
     sysm = make_config("si216")
     x = sysm.psi0()
-    dev = RsdftGpu(sysm, precision=PREC_MIXED_FP32)
+    dev = RsdftGpu(sysm, precision=PREC_MIXED_BF16)
     dev.attach(x)
     dev.gram_schmidt(x)
     dev.sweep(x, 1)

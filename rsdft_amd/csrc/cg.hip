@@ -449,23 +449,25 @@ k_cg_update(T *HX, const T *HP, T *X, const T *P, T *G, CGView v, long ML, int n
 
 // x = u1 x + u2 p unless rejected (cg_module.f90:366-368); for the bands
 // k_cg_update already moved, a rejection restores x = (x_new - u2 p) / u1.
-// One workgroup per band: almost every band has nothing to do here.
+// Same chunk x band grid as the other element-wise kernels: almost every
+// band has nothing to do here and its workgroups return at once.
 template <class T>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kChunkThreads)
 k_cg_accept(T *X, const T *P, CGView v, long ML)
 {
-    const int b = blockIdx.x;
+    BAND_LOOP_BEGIN
     if (!v.active[b]) return;
     const bool spec = cg_speculative(v.u1[b]);
     if (spec == (v.accept[b] != 0)) return;
+    typedef typename Vec16<T>::type V;
+    constexpr int VW = kVW<T>;
     const T u1 = (T)v.u1[b], u2 = (T)v.u2[b];
-    const long off = (long)b * ML;
-    if (spec) {
-        for (long i = threadIdx.x; i < ML; i += blockDim.x)
-            X[off + i] = (X[off + i] - u2 * P[off + i]) / u1;
-    } else {
-        for (long i = threadIdx.x; i < ML; i += blockDim.x)
-            X[off + i] = u1 * X[off + i] + u2 * P[off + i];
+    for (long i = i0 + (long)threadIdx.x * VW; i < i1; i += (long)kChunkThreads * VW) {
+        V x = *reinterpret_cast<const V *>(X + off + i);
+        const V pv = *reinterpret_cast<const V *>(P + off + i);
+#pragma unroll
+        for (int e = 0; e < VW; ++e) x[e] = spec ? (x[e] - u2 * pv[e]) / u1 : u1 * x[e] + u2 * pv[e];
+        *reinterpret_cast<V *>(X + off + i) = x;
     }
 }
 
@@ -692,7 +694,7 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
                            nch, part);
         collapse_parts(part, nb, nch);
         hipLaunchKernelGGL(k_cg_reject, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV);
-        hipLaunchKernelGGL((k_cg_accept<T>), dim3(nb), dim3(1024), 0, c.stream, X, P, v, ML);
+        hipLaunchKernelGGL((k_cg_accept<T>), egrid, eblk, 0, c.stream, X, P, v, ML);
     }
     hipLaunchKernelGGL(k_cg_final, bgrid, bblk, 0, c.stream, v, nb, esp_dev, res_dev);
     RS_HIP(hipGetLastError());

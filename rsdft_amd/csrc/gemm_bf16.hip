@@ -516,20 +516,33 @@ k_gemm_tn3(const float *__restrict__ A, const float *__restrict__ B, long K, int
         }
 }
 
-// C(i,j) = alpha * sum_chunk W[chunk](i,j); symmetric: tiles below the
-// diagonal were not computed, take them from the mirrored element.
-__global__ void k_gemm_tn_reduce(const double *__restrict__ W, int nb, int nchunk, double alpha,
-                                 int sym, int tile, double *__restrict__ C)
+// C(i,j) = alpha * sum_chunk W[chunk](i,j), summed in chunk order, one
+// element per thread.  A workgroup covers 32 rows x 8 columns of C and reads
+// W along the columns (coalesced).  Symmetric products: the GEMM computed
+// only the tile pairs ti <= tj; a block inside such a pair also writes its
+// transpose when the pair is off the diagonal (through LDS, as 64-byte runs),
+// and the blocks below the diagonal tiles return.
+__global__ void __launch_bounds__(256)
+k_gemm_tn_reduce(const double *__restrict__ W, int nb, int nchunk, double alpha, int sym,
+                 int tile, double *__restrict__ C)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    if (i >= nb) return;
-    int si = i, sj = j;
-    if (sym && (i / tile) > (j / tile)) { si = j; sj = i; }
-    const long off = (long)sj * nb + si, stride = (long)nb * nb;
+    __shared__ double tr[8][33];
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 8;
+    if (sym && i0 / tile > j0 / tile) return;
+    const bool mirror = sym && i0 / tile < j0 / tile;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int i = i0 + tx, j = j0 + ty;
     double s = 0.0;
-    for (int c = 0; c < nchunk; ++c) s += W[c * stride + off];
-    C[(long)j * nb + i] = alpha * s;
+    if (i < nb && j < nb) {
+        const long off = (long)j * nb + i, stride = (long)nb * nb;
+        for (int c = 0; c < nchunk; ++c) s += W[c * stride + off];
+        C[(long)j * nb + i] = alpha * s;
+    }
+    if (!mirror) return;
+    tr[ty][tx] = alpha * s;
+    __syncthreads();
+    const int a = threadIdx.x & 7, b = threadIdx.x >> 3;   // C(j0 + a, i0 + b) = C(i0 + b, j0 + a)
+    if (j0 + a < nb && i0 + b < nb) C[(long)(i0 + b) * nb + j0 + a] = tr[a][b];
 }
 
 // ---------------------------------------------------------------------------
@@ -753,6 +766,7 @@ static int gemm_tn_split3(const float *A, const float *B, int nb, long K, double
                           double *C, bool sym_result)
 {
     Context &c = ctx();
+    hipStream_t st = c.stream;
     constexpr int TILE = kTn3Tile;
     const int nt = (nb + TILE - 1) / TILE;
     const bool sym = A == B || sym_result;
@@ -773,10 +787,10 @@ static int gemm_tn_split3(const float *A, const float *B, int nb, long K, double
     }
     const long nwork = (long)npairs * nchunk;
     const long grid = (nwork + 7) / 8 * 8;
-    hipLaunchKernelGGL(k_gemm_tn3, dim3(grid), dim3(kTn3Threads), 2 * kTn3Stage, c.stream, A, B,
+    hipLaunchKernelGGL(k_gemm_tn3, dim3(grid), dim3(kTn3Threads), 2 * kTn3Stage, st, A, B,
                        K, nb, nt, sym ? 1 : 0, npairs, kc, nchunk, W);
     RS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_gemm_tn_reduce, dim3((nb + 127) / 128, nb), dim3(128), 0, c.stream, W, nb,
+    hipLaunchKernelGGL(k_gemm_tn_reduce, dim3((nb + 31) / 32, (nb + 7) / 8), dim3(256), 0, st, W, nb,
                        nchunk, alpha, sym ? 1 : 0, TILE, C);
     RS_HIP(hipGetLastError());
     return kOK;
@@ -787,6 +801,7 @@ static int gemm_tn_mode(const T *A, const T *B, int nb, long K, double alpha, do
                         bool sym_result)
 {
     Context &c = ctx();
+    hipStream_t st = c.stream;
     constexpr int BB = 2, TILE = 32 * BB;
     constexpr int KB = sizeof(T) == 8 ? 16 : GKB;
     const int nt = (nb + TILE - 1) / TILE;
@@ -801,10 +816,10 @@ static int gemm_tn_mode(const T *A, const T *B, int nb, long K, double alpha, do
     double *W = c.gemm_part.as<double>();
     long nwork = (long)npairs * nchunk;
     long grid = (nwork + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_gemm_tn<T, MODE, BB>), dim3(grid), dim3(256), 0, c.stream, A, B, K, nb,
+    hipLaunchKernelGGL((k_gemm_tn<T, MODE, BB>), dim3(grid), dim3(256), 0, st, A, B, K, nb,
                        nt, sym ? 1 : 0, npairs, kc, nchunk, W);
     RS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_gemm_tn_reduce, dim3((nb + 127) / 128, nb), dim3(128), 0, c.stream, W, nb,
+    hipLaunchKernelGGL(k_gemm_tn_reduce, dim3((nb + 31) / 32, (nb + 7) / 8), dim3(256), 0, st, W, nb,
                        nchunk, alpha, sym ? 1 : 0, TILE, C);
     RS_HIP(hipGetLastError());
     return kOK;
