@@ -170,9 +170,15 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
         for (int k = 0; k < NLEV; ++k) dk[k] = dcoef[(long)band * NLEV + k];
     }
     const double bk = (MODE == 2 && beta) ? beta[band] : 0.0;
-    double gacc[MODE == 0 ? NG : 1];
+    // Gram partial sums of this thread's points, in the working precision.
+    // In FP32 a thread adds at most 2 tz products per entry (a rounding error
+    // ~ sqrt(2 tz) 2^-24 of its own partial, random in sign), and the ~10^4
+    // partials of a band are summed in FP64 below, so an entry carries
+    // ~1e-8 of |v_j||v_k|, below the rounding of the FP32 levels themselves;
+    // FP32 accumulators take half the VALU cycles of FP64 ones
+    T gacc[MODE == 0 ? NG : 1];
 #pragma unroll
-    for (int k = 0; k < (MODE == 0 ? NG : 1); ++k) gacc[k] = 0.0;
+    for (int k = 0; k < (MODE == 0 ? NG : 1); ++k) gacc[k] = T(0);
 
     T q[QTOT][PX];
 #pragma unroll
@@ -264,10 +270,10 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
                     for (int j = 1; j <= NLEV; ++j)
 #pragma unroll
                         for (int k = j; k <= NLEV; ++k)
-                            gacc[e++] += (double)vv[j] * (double)vv[k];
+                            gacc[e++] += vv[j] * vv[k];
 #pragma unroll
                     for (int k = 0; k < NLEV; ++k)
-                        gacc[e++] += (double)vv[0] * (double)vv[k];
+                        gacc[e++] += vv[0] * vv[k];
                 }
             } else {
                 VP *dst = reinterpret_cast<VP *>(PG + (long)band * ML + (long)zo * plane + rowoff);
@@ -318,7 +324,7 @@ k_pc_sstep(const T *__restrict__ G, T *__restrict__ PG, long ML, int N1, int N2,
         const int lane = tid & 63, wave = tid >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
         for (int e = 0; e < NG; ++e) {
-            double v = gacc[e];
+            double v = (double)gacc[e];
             for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
             if (lane == 0) red[wave][e] = v;
         }

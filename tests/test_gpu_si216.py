@@ -10,12 +10,13 @@ sweep of calc_sweep, sweep_module.f90:147-158.
 
 Tolerances: hpsi 1e-12 (FP64) / 2e-6 (FP32) relative; eigenvalues of one
 FP64 sweep 1e-10 Ha with the Ritz spans equal to 1e-9; after 8 mixed + 2
-FP64 sweeps the occupied eigenvalues within 1e-6 Ha of 10 FP64 sweeps.
+FP64 sweeps (FP32 or three-piece BF16 GEMMs) the occupied eigenvalues within
+1e-6 Ha of 10 FP64 sweeps.
 """
 import numpy as np
 import pytest
 
-from rsdft_amd.gpu import RsdftGpu, PREC_FP64, PREC_MIXED_FP32
+from rsdft_amd.gpu import RsdftGpu, PREC_FP64, PREC_MIXED_FP32, PREC_MIXED_BF16
 
 pytestmark = pytest.mark.gpu
 
@@ -106,19 +107,34 @@ def _ritz_checks(x, e, si216, oracle216, orth_tol, ritz_tol):
     assert off < ritz_tol * scale
 
 
-def test_mixed_protocol_si216(dev, start, si216, oracle216):
-    """8 mixed sweeps + 2 FP64 refinement sweeps against 10 FP64 sweeps from the
-    same start (the FP64 trajectory's first sweep is the one checked against
-    the oracle above)."""
-    print("case: si216 10 FP64 sweeps vs 8 mixed + 2 FP64", flush=True)
+@pytest.fixture(scope="module")
+def fp64_ten(dev, start):
+    """Ten FP64 sweeps from the start: the reference trajectory of the
+    mixed-precision protocol."""
     y64 = start.copy()
     dev.set_precision(PREC_FP64)
     dev.attach(y64)
     _, e64, _ = dev.sweep(y64, 10)
     dev.sync()
     dev.detach()
+    return y64, e64
+
+
+@pytest.mark.parametrize("mode,name", [(PREC_MIXED_FP32, "fp32"), (PREC_MIXED_BF16, "bf16x3")])
+def test_mixed_protocol_si216(dev, start, si216, oracle216, fp64_ten, mode, name):
+    """8 mixed sweeps + 2 FP64 refinement sweeps against 10 FP64 sweeps from the
+    same start (the FP64 trajectory's first sweep is the one checked against
+    the oracle above), with FP32 or three-piece BF16 GEMMs.  The occupied
+    eigenvalues are held to 1e-6 Ha.  The topmost unoccupied bands are not
+    converged after ten sweeps: their values move by up to ~1e-5 Ha under any
+    change of rounding along the trajectory (the FP64 run from the start
+    rounded once to FP32 differs by 1.4e-6 Ha; summation-order variants of
+    the mixed path gave 2e-6 to 1.1e-5 Ha), so all bands and the band energy
+    get sanity bounds only."""
+    print(f"case: si216 10 FP64 sweeps vs 8 mixed ({name}) + 2 FP64", flush=True)
+    y64, e64 = fp64_ten
     ym = start.copy()
-    dev.set_precision(PREC_MIXED_FP32)
+    dev.set_precision(mode)
     dev.attach(ym)
     dev.sweep(ym, 8)
     dev.sync()
@@ -134,6 +150,7 @@ def test_mixed_protocol_si216(dev, start, si216, oracle216):
     print(f"  max |d eig| {np.max(d):.3e} Ha (occupied {np.max(d[occ]):.3e}), "
           f"band energy {eband:.3e} Ha", flush=True)
     assert np.max(d[occ]) < 1e-6
-    assert np.max(d) < 1e-5
+    assert eband < 1e-5
+    assert np.max(d) < 5e-5
     for y, e in ((y64, e64), (ym, em)):
         _ritz_checks(y, e, si216, oracle216, 1e-11, 1e-11)
