@@ -626,11 +626,95 @@ static int ensure_dense(int nb)
 }
 
 // S <- R^-1 in place, S = dV X^T X = R^T R on entry (upper Cholesky factor)
+// ---------------------------------------------------------------------------
+// Upper Cholesky S = R^T R (dpotrf('U')) blocked by kPotfBlk columns: each
+// diagonal block is factored by one workgroup with the block in LDS (right-
+// looking, the reciprocal of the pivot scaling row j as dpotf2 does), then
+// the block row to its right is solved (dtrsm, R12 = R11^-T S12) and the
+// trailing matrix updated (dsyrk, S22 -= R12^T R12) by rocBLAS.  *info = j+1
+// for the first non-positive pivot j (as dpotrf), and the factorisation stops.
+// ---------------------------------------------------------------------------
+constexpr int kPotfBlk = 120;
+constexpr int kPotfThreads = 1024;
+
+__global__ void k_clear_int(int *p) { *p = 0; }
+
+__global__ void __launch_bounds__(kPotfThreads)
+k_potf2_lds(double *__restrict__ A, int lda, int kb, int k0, int *__restrict__ info)
+{
+    extern __shared__ double Lb[];   // Lb[c * (kb + 1) + r] = A(r, c)
+    __shared__ int bad;
+    const int P = kb + 1, tid = threadIdx.x, nt = blockDim.x;
+    constexpr int TX = 32, TY = kPotfThreads / 32;
+    if (*info != 0) return;          // an earlier block failed (uniform)
+    for (int e = tid; e < kb * kb; e += nt) {
+        const int col = e / kb, r = e - col * kb;
+        Lb[col * P + r] = A[(long)col * lda + r];
+    }
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    int jend = kb;
+    for (int j = 0; j < kb; ++j) {
+        const double d = Lb[j * P + j];
+        if (!(d > 0.0)) {            // the same value in every thread
+            if (tid == 0) bad = k0 + j + 1;
+            jend = j;
+            break;
+        }
+        const double rjj = sqrt(d), inv = 1.0 / rjj;
+        __syncthreads();             // every thread has read the pivot
+        for (int k = j + tid; k < kb; k += nt) Lb[k * P + j] = k == j ? rjj : Lb[k * P + j] * inv;
+        __syncthreads();
+        // trailing update on a 32 x TY thread grid (no index division)
+        for (int k = j + 1 + tid / TX; k < kb; k += TY) {
+            const double rk = Lb[k * P + j];
+            for (int i = j + 1 + tid % TX; i <= k; i += TX) Lb[k * P + i] -= Lb[i * P + j] * rk;
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < kb * kb; e += nt) {
+        const int col = e / kb, r = e - col * kb;
+        if (r <= col && col < jend) A[(long)col * lda + r] = Lb[col * P + r];
+    }
+    if (tid == 0 && bad) *info = bad;
+}
+
+static int potrf_upper(double *S, int n, int *info)
+{
+    Context &c = ctx();
+    static bool lds_set = false;
+    const size_t lds = sizeof(double) * (size_t)kPotfBlk * (kPotfBlk + 1);
+    if (!lds_set) {
+        RS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_potf2_lds),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        lds_set = true;
+    }
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(1), 0, c.stream, info);
+    const double one = 1.0, mone = -1.0;
+    for (int k0 = 0; k0 < n; k0 += kPotfBlk) {
+        const int kb = std::min(kPotfBlk, n - k0), rest = n - k0 - kb;
+        double *A11 = S + (long)k0 * n + k0;
+        hipLaunchKernelGGL(k_potf2_lds, dim3(1), dim3(kPotfThreads), lds, c.stream, A11, n, kb, k0,
+                           info);
+        RS_HIP(hipGetLastError());
+        if (rest > 0) {
+            double *A12 = S + (long)(k0 + kb) * n + k0;
+            double *A22 = S + (long)(k0 + kb) * n + k0 + kb;
+            RS_BLAS(rocblas_dtrsm(c.blas, rocblas_side_left, rocblas_fill_upper,
+                                  rocblas_operation_transpose, rocblas_diagonal_non_unit, kb, rest,
+                                  &one, A11, n, A12, n));
+            RS_BLAS(rocblas_dsyrk(c.blas, rocblas_fill_upper, rocblas_operation_transpose, rest, kb,
+                                  &mone, A12, n, &one, A22, n));
+        }
+    }
+    return kOK;
+}
+
 static int chol_inverse(double *S, int nb)
 {
     Context &c = ctx();
     int *info = c.einfo.as<int>();
-    RS_BLAS(rocsolver_dpotrf(c.blas, rocblas_fill_upper, nb, S, nb, info));
+    RS_TRY(potrf_upper(S, nb, info));
     hipLaunchKernelGGL(k_check_info, dim3(1), dim3(1), 0, c.stream, info, info + 2);
     RS_BLAS(rocsolver_dtrtri(c.blas, rocblas_fill_upper, rocblas_diagonal_non_unit, nb, S, nb,
                              info + 1));
