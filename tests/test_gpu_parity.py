@@ -60,6 +60,32 @@ def test_gram_schmidt_fp64(dev, oracle_si8, si8):
     assert rel(q, qo) < 1e-11
 
 
+def test_gram_schmidt_fp64_many_blocks(dev, oracle_si8, si8):
+    """More bands than one 120-column diagonal block of the blocked Cholesky
+    (the block rows go through dtrsm and dsyrk): 130 bands on the Si8 grid."""
+    print("case: gram_schmidt fp64 si8 130 bands", flush=True)
+    psi = si8.psi0(0, 130)
+    q = dev.gram_schmidt(psi)
+    qo = oracle_si8.gram_schmidt(psi)
+    assert np.max(np.abs(q @ q.T * si8.grid.dV - np.eye(psi.shape[0]))) < 1e-12
+    assert rel(q, qo) < 1e-10
+
+
+def test_gram_schmidt_zero_band_is_refused(dev, si8):
+    """A zero band makes the overlap singular: its pivot of the Cholesky
+    factorisation is exactly zero, which the library reports as an error (the
+    reference's dpotrf returns info = j + 1 there) instead of returning
+    non-orthonormal vectors."""
+    from rsdft_amd.gpu import RsdftGpuError
+    print("case: gram_schmidt fp64 si8 zero band", flush=True)
+    psi = si8.psi0()
+    psi[5] = 0.0
+    with pytest.raises(RsdftGpuError, match="Cholesky"):
+        dev.gram_schmidt(psi)
+    q = dev.gram_schmidt(si8.psi0())   # the context is usable afterwards
+    assert np.max(np.abs(q @ q.T * si8.grid.dV - np.eye(q.shape[0]))) < 1e-12
+
+
 @pytest.mark.parametrize("eig_mode", [1, 0, 2, 3])
 def test_subspace_diag_fp64(dev, oracle_si8, si8, eig_mode):
     """eig_mode 1 (refinement, here falling back to mode 3: Hsub of random
