@@ -308,6 +308,7 @@ int rsdft_gpu_init(int N1, int N2, int N3, const double *Hgrid, double dV, int M
         RS_BLAS(rocblas_set_atomics_mode(c.blas, rocblas_atomics_not_allowed));
     }
     RS_TRY(bind_scratch(c.device));
+    c.refine_skip = c.refine_fails = 0;
     c.g = GridDesc();
     c.g.N1 = N1;
     c.g.N2 = N2;
@@ -620,6 +621,7 @@ int rsdft_gpu_set_eigensolver(int mode)
 {
     if (mode < 0 || mode > 3) return set_error(kErrArg, "eigensolver mode %d", mode);
     ctx().eig_mode = mode;
+    ctx().refine_skip = ctx().refine_fails = 0;
     return kOK;
 }
 
@@ -1050,9 +1052,13 @@ int rsdft_gpu_test_eigh(const double *H, int n, int mode, double *w, double *V, 
     RS_HIP(hipEventCreate(&b));
     const int saved = c.eig_mode;
     c.eig_mode = mode;
+    const int saved_skip = c.refine_skip, saved_fails = c.refine_fails;
+    c.refine_skip = c.refine_fails = 0;   // the test always attempts what mode names
     RS_HIP(hipEventRecord(a, c.stream));
     const int rc = eigh_sub(A, n, Vb, D, &Vec);
     c.eig_mode = saved;
+    c.refine_skip = saved_skip;
+    c.refine_fails = saved_fails;
     RS_TRY(rc);
     RS_HIP(hipEventRecord(b, c.stream));
     RS_HIP(hipMemcpyAsync(w, D, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
@@ -1151,6 +1157,7 @@ int rsdft_gpu_finalize(void)
     c.eig_mode = 1;
     c.sweep_mode = 1;
     c.eig_refined = 0;
+    c.refine_skip = c.refine_fails = 0;
     for (double &x : c.timers.ms) x = 0.0;
     c.hpsi_bands = 0;
     return kOK;

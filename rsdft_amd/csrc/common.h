@@ -187,6 +187,11 @@ struct Context {
     int eig_mode = 1;  // 0: dsyevd, 1: refinement when Hsub is nearly diagonal, else 3; 2: own tridiagonal reduction + dstedc; 3: own reduction + bisection / inverse iteration
     int sweep_mode = 1;  // 1: Rayleigh-Ritz through the Cholesky factor and CG's H x, 0: GS then subspace_diag
     long eig_refined = 0;
+    // eig_mode 1: after a refinement attempt fails (the Ritz vectors still
+    // move) the next refine_skip eigensolves go to the tridiagonal path
+    // directly; the skip doubles per consecutive failure (at most 8) and
+    // resets on a success or a change of eigensolver mode
+    int refine_skip = 0, refine_fails = 0;
 
     DevMem vloc64, vloc32;
     NonlocalTables nl;

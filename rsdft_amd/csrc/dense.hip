@@ -735,10 +735,18 @@ int eigh_sub(double *H, int nb, double *Vbuf, double *D, double **Vec)
     int *info = c.einfo.as<int>();
     int refined = 0, done = 0;
     *Vec = H;
-    if (c.eig_mode == 1) {
+    if (c.eig_mode == 1 && c.refine_skip > 0) {
+        --c.refine_skip;
+    } else if (c.eig_mode == 1) {
         RS_TRY(refine_eigh(H, nb, Vbuf, D, &refined));
         c.eig_refined += refined;
         done = refined;
+        if (refined) {
+            c.refine_fails = 0;
+        } else {
+            c.refine_fails = std::min(c.refine_fails + 1, 4);
+            c.refine_skip = 1 << (c.refine_fails - 1);
+        }
     }
     if (!done && c.eig_mode != 0)
         RS_TRY(eigh_tridiag(H, nb, Vbuf, D, info, &done, c.eig_mode == 2 ? 0 : 1));
