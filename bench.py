@@ -43,6 +43,8 @@ MFMA_PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6, "bf16": 2500.0}    # dense MFMA, 
 # mixed run replaces the first 8 by mixed-precision sweeps)
 NSWEEP = 10
 NMIXED = 8
+NCONV = 40       # sweeps of the near-convergence comparison
+NSWEEP_MID = 30  # the FP64 run's drift from here to NCONV marks the bands still moving
 EIG_TOL_HA = 1e-6
 
 
@@ -408,6 +410,12 @@ def main():
     espm = run(x0, NMIXED, NSWEEP - NMIXED)
     # sensitivity floor: FP64 from the start rounded once to FP32
     espp = run(x0.astype(np.float32).astype(np.float64), 0, NSWEEP)
+    # the same comparison near convergence: NCONV FP64 sweeps against
+    # NCONV - 2 mixed + 2 FP64, and the FP64 run's own drift from NSWEEP_MID
+    # to NCONV sweeps (the bands still moving at NCONV)
+    esp64c = run(x0, 0, NCONV)
+    esp64m = run(x0, 0, NSWEEP_MID)
+    espmc = run(x0, NCONV - 2, 2)
     dev.finalize()
     occ = sysm.occ
 
@@ -417,6 +425,22 @@ def main():
 
     eig_err, eig_err_occ, eband_err = errs(espm)
     flo_err, flo_err_occ, flo_eband = errs(espp)
+    dc = np.abs(espmc - esp64c)
+    drift = np.abs(esp64c - esp64m)
+    conv = {
+        "protocol": f"{NCONV} FP64 sweeps vs {NCONV - 2} mixed + 2 FP64 sweeps from the same start",
+        "max_eig_err_ha": float(np.max(dc)),
+        "max_eig_err_occupied_ha": float(np.max(dc[occ > 0])),
+        "band_energy_err_ha": float(abs(np.sum(occ * (espmc - esp64c)))),
+        "bands_over_tol": int(np.sum(dc > 1e-6)),
+        "fp64_drift_ha": {"from_sweeps": NSWEEP_MID, "to_sweeps": NCONV,
+                          "max": float(np.max(drift)), "occupied": float(np.max(drift[occ > 0])),
+                          "bands_over_tol": int(np.sum(drift > 1e-6))},
+    }
+    log(f"after {NCONV} sweeps: max eig err {conv['max_eig_err_ha']:.3e} Ha (occupied "
+        f"{conv['max_eig_err_occupied_ha']:.3e}), band energy {conv['band_energy_err_ha']:.3e}, "
+        f"{conv['bands_over_tol']} bands over 1e-6; FP64 drift {NSWEEP_MID}->{NCONV}: "
+        f"{conv['fp64_drift_ha']['max']:.3e} ({conv['fp64_drift_ha']['bands_over_tol']} bands over 1e-6)")
     d = np.abs(espm - esp64)
     worst = np.argsort(d)[::-1][:5]
     log(f"after {NSWEEP} sweeps ({NMIXED} mixed + {NSWEEP - NMIXED} FP64 vs {NSWEEP} FP64): max "
@@ -518,6 +542,7 @@ def main():
                 "fp64_rounded_start_eig_err_occupied_ha": flo_err_occ,
                 "fp64_rounded_start_band_energy_err_ha": flo_eband,
             },
+            "accuracy_near_convergence": conv,
             "fp64_first_sweep_vs_oracle_max_eig_err_ha":
                 None if esp_oracle is None else float(np.max(np.abs(esp64_1 - esp_oracle))),
             "hpsi_ms_fp32": t_h,
