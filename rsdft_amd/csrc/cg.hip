@@ -415,6 +415,8 @@ k_cg_newp(const T *PG, T *P, CGView v, long ML, int first, const int *mask)
 // saves a pass over x and p.
 __device__ __forceinline__ bool cg_speculative(double u1) { return u1 > 0.05; }
 
+// G == null: the gradient is only summed (the last CG step, whose gradient
+// no later step reads)
 template <class T>
 __global__ void __launch_bounds__(kChunkThreads)
 k_cg_update(T *HX, const T *HP, T *X, const T *P, T *G, CGView v, long ML, int nchunk,
@@ -441,7 +443,7 @@ k_cg_update(T *HX, const T *HP, T *X, const T *P, T *G, CGView v, long ML, int n
             s[0] += (double)g[e] * (double)g[e];
         }
         *reinterpret_cast<V *>(HX + off + i) = hx;
-        *reinterpret_cast<V *>(G + off + i) = g;
+        if (G) *reinterpret_cast<V *>(G + off + i) = g;
         if (spec) *reinterpret_cast<V *>(X + off + i) = xn;
     }
     block_sum_write<1>(s, part + (long)b * nchunk + chunk, 0);
@@ -690,8 +692,8 @@ int cg_batched(T *X, int nb, double *esp_dev, double *res_dev)
             collapse_parts(part, 6L * nb, nch);
             hipLaunchKernelGGL(k_cg_2x2, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV);
         }
-        hipLaunchKernelGGL((k_cg_update<T>), egrid, eblk, 0, c.stream, HX, HP, X, P, G, v, ML,
-                           nch, part);
+        hipLaunchKernelGGL((k_cg_update<T>), egrid, eblk, 0, c.stream, HX, HP, X, P,
+                           icg == c.ncg ? (T *)nullptr : G, v, ML, nch, part);
         collapse_parts(part, nb, nch);
         hipLaunchKernelGGL(k_cg_reject, bgrid, bblk, 0, c.stream, v, part, nb, nch, dV);
         hipLaunchKernelGGL((k_cg_accept<T>), egrid, eblk, 0, c.stream, X, P, v, ML);
