@@ -18,7 +18,9 @@
 //           to ~2^-25 relative, the accuracy of the FP32 products of mode 1,
 //           at six BF16 instructions per 32 k where mode 1 takes eight FP32
 //           instructions of 16x the cycles per flop.
-// NN is applied in correction form, Out = A diag(U) + A (U - diag(U)): the
+// (The FP64 path's TN products use the same split-K kernel with
+// v_mfma_f64_16x16x4f64 and FP64 accumulators, MODE 0.)
+// Mode 2 applies NN in correction form, Out = A diag(U) + A (U - diag(U)): the
 // diagonal is an exact FP32 column scale and only the off-diagonal part goes
 // through the matrix cores, so the product error scales with the
 // off-diagonal of U, which is small for the near-diagonal rotations (R^-1,
