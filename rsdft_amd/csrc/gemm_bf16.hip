@@ -120,20 +120,9 @@ k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int 
     static_assert(QPT * 256 == TILE * QR, "whole quads per thread");
     typedef T Q __attribute__((ext_vector_type(4)));
     typedef typename std::conditional<F64, v4d, v4f>::type Acc;
-    // MODE 2 stages the three BF16 pieces of every element (split once, when
-    // the loading thread stores it): piece arrays [col][k] with a 40-element
-    // (80-byte) pitch, in place of the FP32 copies of the other modes
-    constexpr bool SPL = MODE == 2;
-    constexpr int LDKB = KB + 8;
-    constexpr int PIECE = TILE * LDKB;   // bf16 elements per piece array
-    constexpr size_t STAGE_BYTES = SPL ? 2 * 3 * PIECE * sizeof(__bf16) : 2 * TILE * LDK * sizeof(T);
-    __shared__ __attribute__((aligned(32))) unsigned char lds_raw[2 * STAGE_BYTES];
-    T (*As)[TILE * LDK] = reinterpret_cast<T (*)[TILE * LDK]>(lds_raw);
-    T (*Bs)[TILE * LDK] = reinterpret_cast<T (*)[TILE * LDK]>(lds_raw + 2 * TILE * LDK * sizeof(T));
-    // piece q (0 hi, 1 mid, 2 lo) of matrix m (0 A, 1 B) in stage buffer b
-    auto piece = [&](int b, int m, int q) {
-        return reinterpret_cast<__bf16 *>(lds_raw + b * STAGE_BYTES) + (m * 3 + q) * PIECE;
-    };
+    static_assert(MODE == 0 || MODE == 1, "the BF16 split products run in k_gemm_tn3");
+    __shared__ __attribute__((aligned(32))) T As[2][TILE * LDK];
+    __shared__ __attribute__((aligned(32))) T Bs[2][TILE * LDK];
     const long total = (long)gridDim.x;
     const long w = xcd_work(blockIdx.x, total);
     if (w >= (long)npairs * nchunk) return;
@@ -199,34 +188,13 @@ k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int 
             qb[it] = vb;
         }
     };
-    typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
     auto lstore = [&](const Q (&qa)[QPT], const Q (&qb)[QPT], int buf) {
 #pragma unroll
         for (int it = 0; it < QPT; ++it) {
             const int f = tid + it * 256;
             const int col = f / QR, kq = (f - col * QR) * 4;
-            if constexpr (SPL) {
-#pragma unroll
-                for (int m = 0; m < 2; ++m) {
-                    const Q &x = m == 0 ? qa[it] : qb[it];
-                    v4bf h, md, lo;
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const __bf16 hh = (__bf16)x[e];
-                        const float r = x[e] - (float)hh;
-                        const __bf16 mm = (__bf16)r;
-                        h[e] = hh;
-                        md[e] = mm;
-                        lo[e] = (__bf16)(r - (float)mm);
-                    }
-                    *reinterpret_cast<v4bf *>(piece(buf, m, 0) + col * LDKB + kq) = h;
-                    *reinterpret_cast<v4bf *>(piece(buf, m, 1) + col * LDKB + kq) = md;
-                    *reinterpret_cast<v4bf *>(piece(buf, m, 2) + col * LDKB + kq) = lo;
-                }
-            } else {
-                *reinterpret_cast<Q *>(&As[buf][col * LDK + kq]) = qa[it];
-                *reinterpret_cast<Q *>(&Bs[buf][col * LDK + kq]) = qb[it];
-            }
+            *reinterpret_cast<Q *>(&As[buf][col * LDK + kq]) = qa[it];
+            *reinterpret_cast<Q *>(&Bs[buf][col * LDK + kq]) = qb[it];
         }
     };
     if (nst > 0) gload(ra[0], rb[0], kbeg);
@@ -236,26 +204,7 @@ k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int 
         lstore(ra[S], rb[S], S);
         __syncthreads();
         if (s + 2 < nst) gload(ra[S], rb[S], kbeg + (s + 2) * KB);
-        if constexpr (SPL) {
-            // lane (lrow, lq) holds k = 8 lq .. 8 lq + 7 of its row: the
-            // operand layout of v_mfma_f32_16x16x32_bf16 for the stage's 32 k
-            v8bf pa[3][BB], pb[3][BB];
-#pragma unroll
-            for (int q = 0; q < 3; ++q)
-#pragma unroll
-                for (int b = 0; b < BB; ++b) {
-                    pa[q][b] = *reinterpret_cast<const v8bf *>(
-                        piece(S, 0, q) + (ar + b * 16 + lrow) * LDKB + 8 * lq);
-                    pb[q][b] = *reinterpret_cast<const v8bf *>(
-                        piece(S, 1, q) + (bc + b * 16 + lrow) * LDKB + 8 * lq);
-                }
-#pragma unroll
-            for (int bi = 0; bi < BB; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < BB; ++bj)
-                    ac[bi][bj] = mfma_split3(pa[0][bi], pa[1][bi], pa[2][bi], pb[0][bj],
-                                             pb[1][bj], pb[2][bj], ac[bi][bj]);
-        } else {
+        {
             T af[BB][KPL], bf[BB][KPL];
 #pragma unroll
             for (int b = 0; b < BB; ++b) {
@@ -350,7 +299,14 @@ k_gemm_tn(const T *__restrict__ A, const T *__restrict__ B, long K, int nb, int 
 // ---------------------------------------------------------------------------
 constexpr int kTn3Tile = 128;
 constexpr int kTn3Threads = 512;
-constexpr int kTn3Ldk = GKB + 8;   // bf16 pitch of a piece row
+constexpr int kTn3Ldk = GKB;       // bf16 pitch of a piece row (64 bytes, chunks swizzled)
+
+// LDS position of the 16-byte chunk c (k = 8c .. 8c+7) of row r of a piece:
+// chunk c ^ ((r >> 2) & 3).  Rows are 64 bytes, so a store of 8 rows x 4
+// chunks fills the 64 banks exactly twice (the least for 512 bytes), and the
+// 16 lanes that read one chunk of 16 consecutive rows meet 16 different
+// 4-bank groups.
+__device__ __forceinline__ int tn3_swz(int r, int c) { return c ^ ((r >> 2) & 3); }
 constexpr size_t kTn3Stage = 2 * 3 * (size_t)kTn3Tile * kTn3Ldk * sizeof(__bf16);
 
 __global__ void __launch_bounds__(kTn3Threads)
@@ -390,6 +346,9 @@ k_gemm_tn3(const float *__restrict__ A, const float *__restrict__ B, long K, int
     const int ar = (wave & 3) * 32, bc = (wave >> 2) * 64;
     const int lrow = lane & 15, lq = lane >> 4;
     const bool vec = (K & 3) == 0;
+    // the lane's chunk lq of rows 16 m + lrow: ar, bc and 16 b are multiples
+    // of 16, so tn3_swz depends on lrow only
+    const int rchunk = 8 * tn3_swz(lrow, lq);
 
     double accd[BI][BJ][4];
 #pragma unroll
@@ -441,9 +400,10 @@ k_gemm_tn3(const float *__restrict__ A, const float *__restrict__ B, long K, int
                     md[e] = mm;
                     lo[e] = (__bf16)(r - (float)mm);
                 }
-                *reinterpret_cast<v4bf *>(piece(buf, m, 0) + col * LDKB + kq) = h;
-                *reinterpret_cast<v4bf *>(piece(buf, m, 1) + col * LDKB + kq) = md;
-                *reinterpret_cast<v4bf *>(piece(buf, m, 2) + col * LDKB + kq) = lo;
+                const int at = col * LDKB + 8 * tn3_swz(col, kq >> 3) + (kq & 7);
+                *reinterpret_cast<v4bf *>(piece(buf, m, 0) + at) = h;
+                *reinterpret_cast<v4bf *>(piece(buf, m, 1) + at) = md;
+                *reinterpret_cast<v4bf *>(piece(buf, m, 2) + at) = lo;
             }
         }
     };
@@ -466,13 +426,15 @@ k_gemm_tn3(const float *__restrict__ A, const float *__restrict__ B, long K, int
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
 #pragma unroll
-            for (int b = 0; b < BI; ++b)
+            for (int b = 0; b < BI; ++b) {
                 pa[q][b] = *reinterpret_cast<const v8bf *>(piece(S, 0, q) +
-                                                           (ar + b * 16 + lrow) * LDKB + 8 * lq);
+                                                           (ar + b * 16 + lrow) * LDKB + rchunk);
+            }
 #pragma unroll
-            for (int b = 0; b < BJ; ++b)
+            for (int b = 0; b < BJ; ++b) {
                 pb[q][b] = *reinterpret_cast<const v8bf *>(piece(S, 1, q) +
-                                                           (bc + b * 16 + lrow) * LDKB + 8 * lq);
+                                                           (bc + b * 16 + lrow) * LDKB + rchunk);
+            }
         }
         if (s + 1 < nst) {
             lstore(ra[1 - S], rb[1 - S], 1 - S);
@@ -586,6 +548,14 @@ __global__ void k_split_u3(const double *__restrict__ U, int nb, int kpad,
     piece[2 * n + t] = (__bf16)(r - (float)m);
 }
 
+// LDS position of the 16-byte chunk c (k = 8c .. 8c+7) of row r of an A
+// piece: chunk c ^ ((r >> 4) & 3).  The loading threads store rows 4 apart
+// (80-byte row pitch): unswizzled, rows 16 apart land on the same banks (a
+// 4-way conflict on every store); with the swizzle the 64 lanes of a store
+// cover the 64 banks once, and the 16 lanes that read one chunk of 16
+// consecutive rows still do.
+__device__ __forceinline__ int nn_swz(int r, int c) { return c ^ ((r >> 4) & 3); }
+
 template <int BN>
 __global__ void __launch_bounds__(kNnThreads)
 k_gemm_nn(const float *__restrict__ A, const __bf16 *__restrict__ Up, int kpad, long M, int nb,
@@ -680,8 +650,9 @@ k_gemm_nn(const float *__restrict__ A, const __bf16 *__restrict__ Up, int kpad, 
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
-                *reinterpret_cast<v2bf *>(Ap + q * PA + (rq + e) * LDKB + kp) =
-                    v2bf{pc[q][0], pc[q][1]};
+                const int row = rq + e;
+                *reinterpret_cast<v2bf *>(Ap + q * PA + row * LDKB + 8 * nn_swz(row, kp >> 3) +
+                                          (kp & 7)) = v2bf{pc[q][0], pc[q][1]};
             }
         }
 #pragma unroll
@@ -701,9 +672,11 @@ k_gemm_nn(const float *__restrict__ A, const __bf16 *__restrict__ Up, int kpad, 
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
-                pa[q][b] = *reinterpret_cast<const v8bf *>(
-                    Ap + q * PA + (wr * 32 + b * 16 + lrow) * LDKB + 8 * lq);
+            for (int b = 0; b < 2; ++b) {
+                const int row = wr * 32 + b * 16 + lrow;
+                pa[q][b] = *reinterpret_cast<const v8bf *>(Ap + q * PA + row * LDKB +
+                                                           8 * nn_swz(row, lq));
+            }
 #pragma unroll
             for (int b = 0; b < BN; ++b)
                 pb[q][b] = *reinterpret_cast<const v8bf *>(
