@@ -553,7 +553,8 @@ __global__ void k_split_u3(const double *__restrict__ U, int nb, int kpad,
 // (80-byte row pitch): unswizzled, rows 16 apart land on the same banks (a
 // 4-way conflict on every store); with the swizzle the 64 lanes of a store
 // cover the 64 banks once, and the 16 lanes that read one chunk of 16
-// consecutive rows still do.
+// consecutive rows still do.  The U pieces, copied as whole 16-byte chunks,
+// use the TN layout (64-byte rows, tn3_swz).
 __device__ __forceinline__ int nn_swz(int r, int c) { return c ^ ((r >> 4) & 3); }
 
 template <int BN>
@@ -563,7 +564,8 @@ k_gemm_nn(const float *__restrict__ A, const __bf16 *__restrict__ Up, int kpad, 
 {
     constexpr int TM = kNnRows, TN = 32 * BN, NT = kNnThreads;
     constexpr int LDKB = GKB + 8;                      // bf16 pitch of a piece row
-    constexpr int PA = TM * LDKB, PB = TN * LDKB;      // elements per piece array
+    constexpr int LDKU = GKB;                          // U pieces: 64-byte rows, tn3_swz chunks
+    constexpr int PA = TM * LDKB, PB = TN * LDKU;      // elements per piece array
     constexpr int SIN = 3 * (PA + PB) * 2;             // bytes of the staged pieces
     constexpr int SC = TN * (TM + 4) * 4;              // bytes of the output staging
     constexpr int SMEM = SIN > SC ? SIN : SC;
@@ -589,6 +591,9 @@ k_gemm_nn(const float *__restrict__ A, const __bf16 *__restrict__ Up, int kpad, 
     const long pn = (long)nb * kpad;                   // elements per piece of U
     // A loads: thread t owns rows rq .. rq+3 at k = kb + kp, kb + kp + 1
     const int rq = (tid % (TM / 4)) * 4, kp = (tid / (TM / 4)) * 2;
+    // the lane's U chunk lq of column 16 m + lrow (wn 16 BN and 16 b are
+    // multiples of 16, so tn3_swz depends on lrow only)
+    const int uchunk = 8 * tn3_swz(lrow, lq);
 
     v4f acc[2][BN];
 #pragma unroll
@@ -661,7 +666,8 @@ k_gemm_nn(const float *__restrict__ A, const __bf16 *__restrict__ Up, int kpad, 
             if (f < NU) {
                 const int q = f / (TN * 4), rem = f - q * (TN * 4);
                 const int col = rem >> 2, c8 = (rem & 3) * 8;
-                *reinterpret_cast<uint4 *>(Bp + q * PB + col * LDKB + c8) = tu[u];
+                *reinterpret_cast<uint4 *>(Bp + q * PB + col * LDKU + 8 * tn3_swz(col, c8 >> 3)) =
+                    tu[u];
             }
         }
         __syncthreads();
@@ -680,7 +686,7 @@ k_gemm_nn(const float *__restrict__ A, const __bf16 *__restrict__ Up, int kpad, 
 #pragma unroll
             for (int b = 0; b < BN; ++b)
                 pb[q][b] = *reinterpret_cast<const v8bf *>(
-                    Bp + q * PB + (wn * 16 * BN + b * 16 + lrow) * LDKB + 8 * lq);
+                    Bp + q * PB + (wn * 16 * BN + b * 16 + lrow) * LDKU + uchunk);
         }
         __syncthreads();
 #pragma unroll
